@@ -1,0 +1,157 @@
+// Device-side building blocks of the gfx950 tracker: complex arithmetic,
+// wave64 lane-group primitives, table-driven evaluation and the group LU.
+//
+// A "group" is G consecutive lanes of one wave64 that cooperate on ONE path:
+// lane gl < n owns equation/row gl (the reference's one-thread-per-row idea,
+// GPU-HC/magmaHC/gpu-kernels/kernel_HC_Solver_generalized_3views_4pts.cu:299-300,
+// re-laid for wave64: floor(64/G) paths per wave, G need not be a power of 2).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hcx {
+namespace dev {
+
+template <class T>
+struct cx {
+  T re, im;
+};
+
+template <class T>
+__device__ __forceinline__ cx<T> cmk(T a, T b) {
+  cx<T> z;
+  z.re = a;
+  z.im = b;
+  return z;
+}
+template <class T>
+__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) {
+  return cmk<T>(a.re + b.re, a.im + b.im);
+}
+template <class T>
+__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) {
+  return cmk<T>(a.re - b.re, a.im - b.im);
+}
+template <class T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return cmk<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+// a - b*c
+template <class T>
+__device__ __forceinline__ cx<T> cfnms(cx<T> a, cx<T> b, cx<T> c) {
+  return cmk<T>(a.re - b.re * c.re + b.im * c.im, a.im - b.re * c.im - b.im * c.re);
+}
+// a + b*c
+template <class T>
+__device__ __forceinline__ cx<T> cfma(cx<T> a, cx<T> b, cx<T> c) {
+  return cmk<T>(a.re + b.re * c.re - b.im * c.im, a.im + b.re * c.im + b.im * c.re);
+}
+template <class T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) {
+  return cmk<T>(a.re * s, a.im * s);
+}
+template <class T>
+__device__ __forceinline__ T cabs1(cx<T> a) {
+  return fabs(a.re) + fabs(a.im);
+}
+template <class T>
+__device__ __forceinline__ T cnorm2(cx<T> a) {
+  return a.re * a.re + a.im * a.im;
+}
+// Complex division a/b by Smith's algorithm (R. L. Smith, CACM 5(8), 1962):
+// scales by the larger component of b to avoid overflow in |b|^2.
+template <class T>
+__device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    const T r = b.im / b.re;
+    const T inv = T(1) / (b.re + b.im * r);
+    return cmk<T>((a.re + a.im * r) * inv, (a.im - a.re * r) * inv);
+  } else {
+    const T r = b.re / b.im;
+    const T inv = T(1) / (b.re * r + b.im);
+    return cmk<T>((a.re * r + a.im) * inv, (a.im * r - a.re) * inv);
+  }
+}
+template <class T>
+__device__ __forceinline__ cx<T> crecip(cx<T> b) {
+  return cdiv<T>(cmk<T>(T(1), T(0)), b);
+}
+
+template <class T, class U>
+__device__ __forceinline__ cx<T> ccast(cx<U> a) {
+  return cmk<T>((T)a.re, (T)a.im);
+}
+
+// ------------------------------------------------------------ lane exchange
+// ds_bpermute: every lane reads `v` of lane `src` (0..63) of its own wave.
+__device__ __forceinline__ int lane_read(int v, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, v); }
+__device__ __forceinline__ float lane_read(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_read(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <class T>
+__device__ __forceinline__ cx<T> lane_read(cx<T> v, int src) {
+  return cmk<T>(lane_read(v.re, src), lane_read(v.im, src));
+}
+
+// Orders LDS traffic between the phases of a wave.  Kernels run with
+// 64-thread workgroups (one wave); a wave's LDS instructions execute in issue
+// order, so a wave-scope fence (no memory-counter drain) is enough to keep the
+// compiler from moving LDS reads above the writes they depend on.
+#define HCX_WAVE_SYNC() __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront")
+
+// Group geometry of one lane.
+struct Group {
+  int G;      // lanes per path
+  int gpw;    // full groups per wave
+  int grp;    // group index of this lane
+  int gl;     // lane index inside the group (row / equation)
+  int gbase;  // wave lane of the group's lane 0
+  bool in_grp;// lane belongs to a full group
+};
+
+__device__ __forceinline__ Group make_group(int G, int lane) {
+  Group g;
+  g.G = G;
+  g.gpw = 64 / G;
+  g.grp = lane / G;
+  g.gl = lane - g.grp * G;
+  g.gbase = g.grp * G;
+  g.in_grp = g.grp < g.gpw;
+  return g;
+}
+
+// Sum of v over the lanes of the group, returned to every lane of the group.
+template <class T>
+__device__ __forceinline__ T group_sum(T v, const Group& g, int lane) {
+  T s = v;
+  for (int off = 1; off < g.G; off <<= 1) {
+    const int src = lane + off < 64 ? lane + off : 63;
+    const T o = lane_read(s, src);
+    if (g.gl + off < g.G) s += o;
+  }
+  return lane_read(s, g.gbase);
+}
+
+// ------------------------------------------------------------ device tables
+// One table term: scalar * c[coef] * prod x[var_q]; var index n reads the
+// constant 1 stored at x[n] (reference table convention, Appendix A format).
+struct __align__(16) Term {
+  int s;
+  int c;
+  uint32_t v03;  // variable indices 0..3, 8 bits each
+  uint32_t v47;  // variable indices 4..7
+};
+
+__device__ __forceinline__ int term_var(const Term& t, int q) {
+  return q < 4 ? (int)((t.v03 >> (8 * q)) & 0xffu) : (int)((t.v47 >> (8 * (q - 4))) & 0xffu);
+}
+
+}  // namespace dev
+}  // namespace hcx

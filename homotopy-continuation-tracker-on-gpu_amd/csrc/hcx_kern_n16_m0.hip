@@ -1,0 +1,16 @@
+// Tracker kernel instances for NMAX = 16, FP64 mode.
+#include "hcx_kernel.hpp"
+
+namespace hcx {
+
+KernelFn kernel_n16_m0(int deg) {
+  switch (deg) {
+    case 2: return track_kernel<16, 2, 0>;
+    case 3: return track_kernel<16, 3, 0>;
+    case 4: return track_kernel<16, 4, 0>;
+    case 6: return track_kernel<16, 6, 0>;
+    default: return nullptr;
+  }
+}
+
+}  // namespace hcx

@@ -1,0 +1,16 @@
+// Tracker kernel instances for NMAX = 8, FP64 mode.
+#include "hcx_kernel.hpp"
+
+namespace hcx {
+
+KernelFn kernel_n8_m0(int deg) {
+  switch (deg) {
+    case 2: return track_kernel<8, 2, 0>;
+    case 3: return track_kernel<8, 3, 0>;
+    case 4: return track_kernel<8, 4, 0>;
+    case 6: return track_kernel<8, 6, 0>;
+    default: return nullptr;
+  }
+}
+
+}  // namespace hcx

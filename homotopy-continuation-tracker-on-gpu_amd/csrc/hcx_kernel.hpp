@@ -1,0 +1,593 @@
+// Tracker kernel templates (shared by the per-instance translation units).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "hcx_device.hpp"
+#include "hcx.h"
+
+namespace hcx {
+
+using dev::cx;
+using dev::cmk;
+
+constexpr int kMaxSeg = 8;
+constexpr int kMaxN = 32;
+
+struct Seg {
+  int n, C, M, K;
+  int hxT, htT;
+  int G;
+  int lds_group;               // bytes of one group's LDS region
+  const dev::Term* hx;         // [n var][hxT][n eq]
+  const dev::Term* ht;         // [htT][n eq]
+  const int* hx_cnt;           // [n] real terms per Jacobian column (max over equations)
+  const double2* start;        // [K][start_stride]
+  int start_stride;
+  int roots_stride;
+  int I;
+  long long P;
+  const double2* chx;          // [I][C+1][M+1]
+  const double2* cht;          // [I][C+1][M]
+  double2* roots;              // [P][roots_stride]
+  uint32_t* flags;
+  int* steps;
+  int* newton;
+  int* refine;
+  double* tfin;
+  double* dtfin;
+  double* resid;
+  uint8_t* conv8;
+  uint8_t* inf8;
+  double2* debug;
+  unsigned long long* inst_done;
+  int* inst_left;
+  unsigned int* counter;
+};
+
+struct Opts {
+  int max_steps, max_corr, incr, refine_iters;
+  double dt0, end_zone, t_tol, dt_min, corr_tol, inf_tol, refine_tol, real_tol;
+};
+
+struct Launch {
+  Seg seg[kMaxSeg];
+  int nseg;
+  int first_block[kMaxSeg + 1];
+  int persistent;
+  Opts o;
+};
+
+namespace dev {
+
+// ------------------------------------------------------------ evaluation
+// Row gl of dH/dx (dev-eval-indxing-generalized_3views_4pts.cuh:70-86).
+template <int NMAX, int DEG, class T>
+__device__ __forceinline__ void eval_jac_row(const Seg& S, int gle, const cx<T>* sx, const cx<T>* sc,
+                                             cx<T> (&A)[NMAX]) {
+  const int n = S.n;
+#pragma unroll
+  for (int j = 0; j < NMAX; ++j) {
+    cx<T> acc = cmk<T>(0, 0);
+    if (j < n) {
+      const int cnt = S.hx_cnt[j];
+      const Term* tp = S.hx + (size_t)j * S.hxT * n + gle;
+#pragma unroll 2
+      for (int k = 0; k < cnt; ++k) {
+        const Term tm = tp[(size_t)k * n];
+        cx<T> v = cscale(sc[tm.c], (T)tm.s);
+#pragma unroll
+        for (int q = 0; q < DEG - 1; ++q) v = cmul(v, sx[term_var(tm, q)]);
+        acc = cadd(acc, v);
+      }
+    }
+    A[j] = acc;
+  }
+}
+
+// Row gl of sum_k s * coef[c] * prod x over the dHdt table: H when coef = c(t),
+// dH/dt when coef = dc/dt (…cuh:89-120).
+template <int DEG, class T>
+__device__ __forceinline__ cx<T> eval_t_row(const Seg& S, int gle, const cx<T>* sx, const cx<T>* coef) {
+  const int n = S.n;
+  cx<T> acc = cmk<T>(0, 0);
+  const Term* tp = S.ht + gle;
+#pragma unroll 2
+  for (int k = 0; k < S.htT; ++k) {
+    const Term tm = tp[(size_t)k * n];
+    cx<T> v = cscale(coef[tm.c], (T)tm.s);
+#pragma unroll
+    for (int q = 0; q < DEG; ++q) v = cmul(v, sx[term_var(tm, q)]);
+    acc = cadd(acc, v);
+  }
+  return acc;
+}
+
+// Coefficient polynomials in t of the group's instance into LDS
+// (eval_parameter_homotopy, …cuh:34-67), computed in FP64 and stored as T.
+template <class T>
+__device__ __forceinline__ void stage_coeffs(const Seg& S, const Group& g, bool act, int inst, double t,
+                                             unsigned char* scb, bool with_dot) {
+  if (!act) return;
+  cx<T>* sc = reinterpret_cast<cx<T>*>(scb);
+  cx<T>* sd = sc + (S.C + 1);
+  const int C1 = S.C + 1, M = S.M;
+  const double2* hx = S.chx + (size_t)inst * C1 * (M + 1);
+  const double2* ht = S.cht + (size_t)inst * C1 * M;
+  for (int j = g.gl; j < C1; j += g.G) {
+    double2 a = hx[(size_t)j * (M + 1) + M];
+    double cr = a.x, ci = a.y;
+    for (int m = M - 1; m >= 0; --m) {
+      const double2 b = hx[(size_t)j * (M + 1) + m];
+      const double nr = cr * t + b.x, ni = ci * t + b.y;
+      cr = nr;
+      ci = ni;
+    }
+    sc[j] = cmk<T>((T)cr, (T)ci);
+    if (with_dot) {
+      double2 d = ht[(size_t)j * M + M - 1];
+      double dr = d.x, di = d.y;
+      for (int m = M - 2; m >= 0; --m) {
+        const double2 b = ht[(size_t)j * M + m];
+        const double nr = dr * t + b.x, ni = di * t + b.y;
+        dr = nr;
+        di = ni;
+      }
+      sd[j] = cmk<T>((T)dr, (T)di);
+    }
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void publish_x(cx<T>* sx, cx<double> x, const Group& g, int n, bool act, bool row) {
+  if (act && row) sx[g.gl] = ccast<T>(x);
+  if (act && g.gl == 0) sx[n] = cmk<T>(T(1), T(0));
+}
+
+template <class T>
+__device__ __forceinline__ T group_max(T v, const Group& g) {
+  for (int s = 1; s < g.G; s <<= 1) {
+    int src = g.gl + s;
+    src = (src >= g.G ? src - g.G : src) + g.gbase;
+    const T o = lane_read(v, src);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// ------------------------------------------------------------ group LU
+// Solves A x = b for the group's n x n system, row r held by lane r (A[j] =
+// A[r][j]).  Returns x_gl to lane gl.  Partial pivoting on cabs1 with the
+// first maximum in current row order (rows carry a position label that is
+// swapped like a physical row interchange); a zero pivot column is skipped
+// and reported through `singular` (dev-cgesv-batched-small.cuh:51-95).
+template <int NMAX, class T>
+__device__ __forceinline__ cx<T> group_lu_solve(cx<T> (&A)[NMAX], cx<T> b, const Group& g, int n, bool row,
+                                                cx<T>* sp, bool& singular) {
+  int label = g.gl;
+  cx<T> own_rinv = cmk<T>(T(0), T(0));  // 1 / U_ii of the row this lane ends up holding
+  uint32_t pl[(NMAX + 3) / 4];
+#pragma unroll
+  for (int q = 0; q < (NMAX + 3) / 4; ++q) pl[q] = 0u;
+  singular = false;
+#pragma unroll
+  for (int k = 0; k < NMAX; ++k) {
+    if (k < n) {
+      T v = (row && label >= k) ? cabs1(A[k]) : T(-1);
+      if (v != v) v = T(INFINITY);
+      int key = (label << 6) | g.gl;
+      for (int s = 1; s < g.G; s <<= 1) {
+        int src = g.gl + s;
+        src = (src >= g.G ? src - g.G : src) + g.gbase;
+        const T ov = lane_read(v, src);
+        const int ok = lane_read(key, src);
+        const bool take = (ov > v) | ((ov == v) & (ok < key));
+        v = take ? ov : v;
+        key = take ? ok : key;
+      }
+      const int bl = key & 63, blabel = key >> 6;
+      const bool zero = !(v > T(0));
+      singular = singular || zero;
+      if (g.gl == bl) {
+#pragma unroll
+        for (int j = k; j < NMAX; ++j)
+          if (j < n) sp[j] = A[j];
+        sp[NMAX] = b;
+      }
+      HCX_WAVE_SYNC();
+      if (g.gl == bl) label = k;
+      else if (label == k) label = blabel;
+      pl[k >> 2] |= (uint32_t)bl << (8 * (k & 3));
+      const cx<T> rinv = zero ? cmk<T>(T(1), T(0)) : crecip(sp[k]);
+      // a zero pivot makes the back substitution produce NaN, as the
+      // reference's division by a zero U_ii does
+      if (g.gl == bl) own_rinv = zero ? cmk<T>(T(NAN), T(NAN)) : rinv;
+      if (row && label > k && !zero) {
+        const cx<T> l = cmul(A[k], rinv);
+#pragma unroll
+        for (int j = k + 1; j < NMAX; ++j)
+          if (j < n) A[j] = cfnms(A[j], l, sp[j]);
+        b = cfnms(b, l, sp[NMAX]);
+      }
+      HCX_WAVE_SYNC();
+    }
+  }
+  // back substitution (dev-cgesv-batched-small.cuh:97-106): x_i from the
+  // lane holding row position i (times its stored 1/U_ii), broadcast, then
+  // eliminated from the rows above
+  cx<T> sol = cmk<T>(T(0), T(0));
+#pragma unroll
+  for (int i = NMAX - 1; i >= 0; --i) {
+    if (i < n) {
+      const int src = g.gbase + (int)((pl[i >> 2] >> (8 * (i & 3))) & 0xffu);
+      const cx<T> q = cmul(b, own_rinv);
+      const cx<T> xi = lane_read(q, src);
+      if (row && label < i) b = cfnms(b, A[i], xi);
+      if (g.gl == i) sol = xi;
+    }
+  }
+  return sol;
+}
+
+// Per-group view of the LDS region.
+template <int NMAX>
+struct Lds {
+  unsigned char* base;
+  __device__ cx<double>* x64() const { return reinterpret_cast<cx<double>*>(base); }
+  __device__ cx<double>* piv64() const { return x64() + (NMAX + 1); }
+  __device__ unsigned char* coef() const { return reinterpret_cast<unsigned char*>(piv64() + (NMAX + 1)); }
+  template <class T>
+  __device__ cx<T>* x() const { return reinterpret_cast<cx<T>*>(base); }
+  template <class T>
+  __device__ cx<T>* piv() const { return reinterpret_cast<cx<T>*>(piv64()); }
+};
+
+// Newton corrector at fixed t in precision T (…4pts.cu:189-223).  The
+// coefficients c(t) must already be in the group's LDS.  `run` enters as the
+// set of groups to correct; ok / inf return the reference's r_isSuccessful /
+// r_isInfFail.
+template <int NMAX, int DEG, class T>
+__device__ __forceinline__ void newton_pass(const Seg& S, const Opts& o, const Group& g, const Lds<NMAX>& L,
+                                            bool run, bool row, cx<double>& x, int& nnewt, bool& ok, bool& inf) {
+  ok = false;
+  inf = false;
+  const int n = S.n;
+  const int gle = row ? g.gl : 0;
+  cx<T>* sx = L.template x<T>();
+  const cx<T>* sc = reinterpret_cast<const cx<T>*>(L.coef());
+  for (int it = 0; it < o.max_corr; ++it) {
+    if (!__any(run)) break;
+    publish_x<T>(sx, x, g, n, run, row);
+    HCX_WAVE_SYNC();
+    cx<T> A[NMAX];
+    eval_jac_row<NMAX, DEG, T>(S, gle, sx, sc, A);
+    const cx<T> h = eval_t_row<DEG, T>(S, gle, sx, sc);
+    bool sing;
+    const cx<T> d = group_lu_solve<NMAX, T>(A, h, g, n, row, L.template piv<T>(), sing);
+    if (run && row) x = csub(x, ccast<double>(d));
+    double rs = row ? (double)cnorm2(d) : 0.0;
+    double rc = row ? cnorm2(x) : 0.0;
+    rs = group_sum(rs, g, threadIdx.x);
+    rc = group_sum(rc, g, threadIdx.x);
+    if (run) {
+      ++nnewt;
+      ok = rs < o.corr_tol * rc;
+      inf = rc > o.inf_tol;
+      if (ok || inf) run = false;
+    }
+  }
+}
+
+}  // namespace dev
+
+using namespace dev;
+
+// One RK4 predictor step in precision T for the groups in `act`
+// (…4pts.cu:139-183): stages at (x0, t), (x0 + h k1, t + h), (x0 + h k2, t + h),
+// (x0 + dt k3, t + dt); returns k1 + 2 k2 + 2 k3 + k4 (component gl).
+// sd_ext: dc/dt table of the group's instance in global memory (constant in t,
+// M = 1) or nullptr to stage dc/dt(t) in LDS next to c(t).
+template <int NMAX, int DEG, class T>
+__device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, const Lds<NMAX>& L, bool act, bool row,
+                                              int gle, int inst, double ts, double dt, cx<double> xl,
+                                              const cx<T>* sd_ext) {
+  const int n = S.n;
+  const double h = 0.5 * dt;
+  cx<double> acc = cmk<double>(0, 0), kprev = cmk<double>(0, 0);
+  cx<T>* sx = L.template x<T>();
+  const cx<T>* sc = reinterpret_cast<const cx<T>*>(L.coef());
+  const cx<T>* sd = sd_ext ? sd_ext : sc + (S.C + 1);
+#pragma unroll 1
+  for (int st = 0; st < 4; ++st) {
+    const double a = st == 0 ? 0.0 : (st == 3 ? dt : h);
+    if (st != 2) stage_coeffs<T>(S, g, act, inst, ts + a, L.coef(), sd_ext == nullptr);
+    const cx<double> xs = cmk<double>(xl.re + a * kprev.re, xl.im + a * kprev.im);
+    publish_x<T>(sx, xs, g, n, act, row);
+    HCX_WAVE_SYNC();
+    cx<T> A[NMAX];
+    eval_jac_row<NMAX, DEG, T>(S, gle, sx, sc, A);
+    const cx<T> bt = eval_t_row<DEG, T>(S, gle, sx, sd);
+    bool sing;
+    const cx<T> k = group_lu_solve<NMAX, T>(A, cmk<T>(-bt.re, -bt.im), g, n, row, L.template piv<T>(), sing);
+    kprev = ccast<double>(k);
+    const double w = (st == 1 || st == 2) ? 2.0 : 1.0;
+    acc = cmk<double>(acc.re + w * kprev.re, acc.im + w * kprev.im);
+  }
+  return acc;
+}
+
+// Lane-group path loop.  MODE 0: everything FP64.  MODE 1 (mixed): FP32
+// predictor and Jacobian, FP32 corrector before the end zone, FP64 corrector
+// in the end zone, FP64 refinement; a path that ends without a refined root
+// is re-tracked from t = 0 entirely in FP64 by the same group (escalation).
+template <int NMAX, int DEG, int MODE>
+__device__ void run_segment(const Launch& LN, int si, unsigned char* smem, int lane) {
+  const Seg& S = LN.seg[si];
+  const Opts& o = LN.o;
+  const Group g = make_group(S.G, lane);
+  const int n = S.n;
+  const bool row = g.in_grp && g.gl < n;
+  const int gle = row ? g.gl : 0;
+  Lds<NMAX> L{smem + (size_t)g.grp * S.lds_group};
+
+  // per-group path state, replicated over the group's lanes
+  long long path = -1;
+  int inst = 0, root = 0;
+  double t = 0.0, dt = 0.0;
+  int cnt = 0, nsteps = 0, asteps = 0, nnewt = 0, nref = 0;
+  bool ez = false, esc = (MODE == 0);
+  int status = 0;  // 0 idle, 1 tracking, 2 finishing
+  uint32_t fl = 0;
+  bool exhausted = !g.in_grp;
+  long long static_path = (long long)((int)blockIdx.x - LN.first_block[si]) * g.gpw + g.grp;
+  // lane-distributed state: component gl of x and of the last accepted x
+  cx<double> x = cmk<double>(0, 0), xl = x;
+
+  auto start_path = [&](bool fp64) {
+    t = 0.0;
+    dt = o.dt0;
+    cnt = 0;
+    asteps = 0;
+    ez = false;
+    esc = fp64;
+    status = 1;
+    if (row) {
+      const double2 v = S.start[(size_t)root * S.start_stride + g.gl];
+      x = cmk<double>(v.x, v.y);
+    } else {
+      x = cmk<double>(0, 0);
+    }
+    xl = x;
+  };
+
+  for (;;) {
+    // ------------------------------------------------ refill idle groups
+    const bool want = status == 0 && !exhausted;
+    if (__any(want)) {
+      long long p;
+      if (LN.persistent) {
+        unsigned int q = 0;
+        if (want && g.gl == 0) q = atomicAdd(S.counter, 1u);
+        p = (long long)(unsigned int)lane_read((int)q, g.gbase);
+      } else {
+        p = static_path;
+      }
+      if (want) {
+        if (p < 0 || p >= S.P) {
+          exhausted = true;
+        } else {
+          path = p;
+          inst = (int)(p / S.K);
+          root = (int)(p - (long long)inst * S.K);
+          nsteps = 0;
+          nnewt = 0;
+          nref = 0;
+          fl = 0;
+          start_path(MODE == 0);
+          if (!LN.persistent) exhausted = true;
+        }
+      }
+    }
+    if (!__any(status != 0)) break;
+
+    // ------------------------------------------------ loop-top tests (:116)
+    bool stepping = false;
+    if (status == 1) {
+      if (!(t < 1.0 && 1.0 - t > o.t_tol)) {
+        status = 2;
+        fl |= HCX_FLAG_TRACKED;
+      } else if (asteps > o.max_steps) {
+        status = 2;
+        fl |= HCX_FLAG_BUDGET;
+      } else {
+        stepping = true;
+      }
+    }
+
+    if (__any(stepping)) {
+      // end zone and step clamp (:121-131)
+      if (stepping) {
+        if (!ez && fabs(1.0 - t) <= o.end_zone + 1e-7) ez = true;
+        if (ez) {
+          if (dt > fabs(1.0 - t)) dt = fabs(1.0 - t);
+        } else if (dt > fabs(1.0 - o.end_zone - t)) {
+          dt = fabs(1.0 - o.end_zone - t);
+        }
+      }
+      const double ts = t;
+      cx<double> acc;
+      if (MODE == 0) {
+        acc = predict<NMAX, DEG, double>(S, g, L, stepping, row, gle, inst, ts, dt, xl, nullptr);
+      } else {
+        const bool p32 = stepping && !esc, p64 = stepping && esc;
+        cx<double> a32 = cmk<double>(0, 0), a64 = a32;
+        if (__any(p32)) a32 = predict<NMAX, DEG, float>(S, g, L, p32, row, gle, inst, ts, dt, xl, nullptr);
+        if (__any(p64)) {
+          HCX_WAVE_SYNC();
+          const cx<double>* sd_ext =
+              S.M == 1 ? reinterpret_cast<const cx<double>*>(S.cht + (size_t)inst * (S.C + 1)) : nullptr;
+          a64 = predict<NMAX, DEG, double>(S, g, L, p64, row, gle, inst, ts, dt, xl, sd_ext);
+        }
+        acc = esc ? a64 : a32;
+      }
+      if (stepping) {
+        const double f = dt / 6.0;
+        x = cmk<double>(xl.re + f * acc.re, xl.im + f * acc.im);
+        t = ts + dt;
+      }
+      // Newton corrector at the new t (:189-223)
+      bool ok = false, inf = false;
+      if (MODE == 0) {
+        newton_pass<NMAX, DEG, double>(S, o, g, L, stepping, row, x, nnewt, ok, inf);
+      } else {
+        const bool run32 = stepping && !ez && !esc, run64 = stepping && (ez || esc);
+        bool ok32 = false, inf32 = false, ok64 = false, inf64 = false;
+        if (__any(run32)) newton_pass<NMAX, DEG, float>(S, o, g, L, run32, row, x, nnewt, ok32, inf32);
+        if (__any(run64)) {
+          HCX_WAVE_SYNC();
+          stage_coeffs<double>(S, g, run64, inst, t, L.coef(), false);
+          newton_pass<NMAX, DEG, double>(S, o, g, L, run64, row, x, nnewt, ok64, inf64);
+        }
+        ok = run64 ? ok64 : ok32;
+        inf = run64 ? inf64 : inf32;
+      }
+      // step control (:225-249)
+      if (stepping) {
+        ++nsteps;
+        ++asteps;
+        if (inf) {
+          status = 2;
+          fl |= HCX_FLAG_INFINITY;
+        } else if (!ok) {
+          dt *= 0.5;
+          x = xl;
+          t = ts;
+          cnt = 0;
+          if (dt < o.dt_min) {
+            status = 2;
+            fl |= HCX_FLAG_MINSTEP;
+          }
+        } else {
+          ++cnt;
+          xl = x;
+          if (cnt >= o.incr) {
+            cnt = 0;
+            dt *= 2.0;
+          }
+        }
+      }
+      HCX_WAVE_SYNC();
+    }
+
+    // ------------------------------------------------ finish: refine + outputs
+    const bool fin = status == 2;
+    if (__any(fin)) {
+      const bool ref = fin && (fl & HCX_FLAG_TRACKED) && o.refine_iters > 0;
+      double resid = 0.0;
+      if (__any(ref)) {
+        stage_coeffs<double>(S, g, ref, inst, 1.0, L.coef(), false);
+        cx<double>* sx = L.template x<double>();
+        const cx<double>* sc = reinterpret_cast<const cx<double>*>(L.coef());
+        bool run = ref, conv = false, sing_any = false;
+        for (int it = 0; it < o.refine_iters; ++it) {
+          if (!__any(run)) break;
+          publish_x<double>(sx, x, g, n, run, row);
+          HCX_WAVE_SYNC();
+          cx<double> A[NMAX];
+          eval_jac_row<NMAX, DEG, double>(S, gle, sx, sc, A);
+          const cx<double> hv = eval_t_row<DEG, double>(S, gle, sx, sc);
+          bool sing;
+          const cx<double> d = group_lu_solve<NMAX, double>(A, hv, g, n, row, L.template piv<double>(), sing);
+          if (run && row) x = csub(x, d);
+          double nd = row ? cnorm2(d) : 0.0, nx = row ? cnorm2(x) : 0.0;
+          nd = group_sum(nd, g, lane);
+          nx = group_sum(nx, g, lane);
+          if (run) {
+            ++nref;
+            sing_any = sing;
+            if (!(nx < o.inf_tol)) {
+              run = false;
+            } else if (sqrt(nd) <= o.refine_tol * fmax(1.0, sqrt(nx))) {
+              conv = true;
+              run = false;
+            }
+          }
+        }
+        // residual max_i |H_i(x, 1)|
+        publish_x<double>(sx, x, g, n, ref, row);
+        HCX_WAVE_SYNC();
+        const cx<double> hv = eval_t_row<DEG, double>(S, gle, sx, sc);
+        double r = row ? sqrt(cnorm2(hv)) : 0.0;
+        if (r != r) r = INFINITY;
+        resid = group_max(r, g);
+        double im = row ? fabs(x.im) : 0.0;
+        if (im != im) im = INFINITY;
+        im = group_max(im, g);
+        if (ref) {
+          if (conv) fl |= HCX_FLAG_REFINED;
+          if (conv && im <= o.real_tol) fl |= HCX_FLAG_REAL;
+          if (sing_any) fl |= HCX_FLAG_SINGULAR;
+        }
+        HCX_WAVE_SYNC();
+      }
+      if (fin) {
+        if (MODE == 1 && !esc && !(fl & HCX_FLAG_REFINED)) {
+          // precision escalation: the mixed attempt did not reach a refined
+          // root; track the same path again from t = 0 in FP64
+          fl = HCX_FLAG_ESCALATED;
+          start_path(true);
+        } else {
+          if (row) S.roots[(size_t)path * S.roots_stride + g.gl] = make_double2(x.re, x.im);
+          if (g.gl == 0) {
+            if (S.roots_stride > n) S.roots[(size_t)path * S.roots_stride + n] = make_double2(1.0, 0.0);
+            S.flags[path] = fl;
+            if (S.steps) S.steps[path] = nsteps;
+            if (S.newton) S.newton[path] = nnewt;
+            if (S.refine) S.refine[path] = nref;
+            if (S.tfin) S.tfin[path] = t;
+            if (S.dtfin) S.dtfin[path] = dt;
+            if (S.resid) S.resid[path] = resid;
+            if (S.conv8) S.conv8[path] = (fl & HCX_FLAG_TRACKED) ? 1 : 0;
+            if (S.inf8) S.inf8[path] = (fl & HCX_FLAG_INFINITY) ? 1 : 0;
+            if (S.debug) S.debug[path] = (fl & HCX_FLAG_TRACKED) ? make_double2(1.0, 0.0) : make_double2(t, dt);
+            if (S.inst_left) {
+              const int left = atomicSub(&S.inst_left[inst], 1);
+              if (left == 1 && S.inst_done) S.inst_done[inst] = wall_clock64();
+            }
+          }
+          status = 0;
+          path = -1;
+        }
+      }
+    }
+  }
+}
+
+template <int NMAX, int DEG, int MODE>
+__global__ __launch_bounds__(64) void track_kernel(const Launch LN) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x;
+  int s = 0;
+  while (s + 1 < LN.nseg && (int)blockIdx.x >= LN.first_block[s + 1]) ++s;
+  for (int tried = 0; tried < LN.nseg; ++tried) {
+    run_segment<NMAX, DEG, MODE>(LN, s, smem, lane);
+    if (!LN.persistent) break;
+    s = s + 1 == LN.nseg ? 0 : s + 1;
+  }
+}
+
+using KernelFn = void (*)(const Launch);
+
+// Kernel instances live in separate translation units (compile in parallel).
+KernelFn kernel_n4_m0(int deg);
+KernelFn kernel_n8_m0(int deg);
+KernelFn kernel_n16_m0(int deg);
+KernelFn kernel_n32_m0(int deg);
+KernelFn kernel_n4_m1(int deg);
+KernelFn kernel_n8_m1(int deg);
+KernelFn kernel_n16_m1(int deg);
+KernelFn kernel_n32_m1(int deg);
+
+}  // namespace hcx

@@ -1,0 +1,812 @@
+// gfx950 path tracker: persistent lane-group kernel, launchers and the
+// tracking half of the C-ABI.
+//
+// Method (per path) restates the reference kernel loop
+// GPU-HC/magmaHC/gpu-kernels/kernel_HC_Solver_generalized_3views_4pts.cu:115-260:
+// end-zone clamp (:121-131), "loopy" RK4 predictor (:139-183), Newton
+// corrector with |dx|^2 < tol |x|^2 success and |x|^2 > 1e14 infinity tests
+// (:189-225), halving / doubling step control (:230-249), outputs (:256-264).
+// Table semantics follow dev-eval-indxing-generalized_3views_4pts.cuh:34-120,
+// the LU follows dev-cgesv-batched-small.cuh:38-107 (cabs1 partial pivoting,
+// first maximum in current row order, multiply-by-reciprocal elimination,
+// division in back substitution).
+//
+// Work decomposition (MI355X-first): a wave64 holds floor(64/G) lane groups;
+// each group tracks one path at a time, lane gl < n owns equation gl.  Groups
+// refill from a device work counter as soon as their path ends, so a wave
+// never idles on its slowest path.  Paths of several problem types
+// ("segments") share one launch; a wave works a segment until its queue is
+// empty and then steals from the others.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <type_traits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "hcx_internal.hpp"
+#include "hcx_kernel.hpp"
+
+namespace hcx {
+
+using dev::cx;
+using dev::cmk;
+
+// Counters of a launch: path queues, per-instance remaining paths, start stamp.
+__global__ void init_kernel(const Launch LN, unsigned long long* batch_start) {
+  const int s = blockIdx.y;
+  if (s >= LN.nseg) return;
+  const Seg& S = LN.seg[s];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *S.counter = 0u;
+  if (S.inst_left && i < S.I) S.inst_left[i] = S.K;
+  if (batch_start && s == 0 && i == 0) *batch_start = wall_clock64();
+}
+
+// Parameter-affine P2C on the device: c_j(p) = A[j][0] + sum_q A[j][1+q] p_q
+// for p0 and p1 of each instance, written as the M = 1 coefficient tables
+// Hx[i][j] = {c(p0), c(p1) - c(p0)}, Ht[i][j] = {c(p1) - c(p0)}, entry C = {1, 0}
+// (the role of Data_Reader::Construct_Coeffs_From_Params, Data_Reader.cpp:59-89).
+__global__ void p2c_affine_kernel(const double2* __restrict__ A, int C, int k, const double2* __restrict__ p0,
+                                  int p0_stride, const double2* __restrict__ p1, int I, double2* __restrict__ hx,
+                                  double2* __restrict__ ht) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j > C || i >= I) return;
+  double2* hxi = hx + ((size_t)i * (C + 1) + j) * 2;
+  double2* hti = ht + ((size_t)i * (C + 1) + j);
+  if (j == C) {
+    hxi[0] = make_double2(1.0, 0.0);
+    hxi[1] = make_double2(0.0, 0.0);
+    hti[0] = make_double2(0.0, 0.0);
+    return;
+  }
+  const double2* a = A + (size_t)j * (1 + k);
+  const double2* q0 = p0 + (size_t)i * p0_stride;
+  const double2* q1 = p1 + (size_t)i * k;
+  double r0 = a[0].x, i0 = a[0].y, r1 = a[0].x, i1 = a[0].y;
+  for (int q = 0; q < k; ++q) {
+    const double2 c = a[1 + q], u = q0[q], v = q1[q];
+    r0 += c.x * u.x - c.y * u.y;
+    i0 += c.x * u.y + c.y * u.x;
+    r1 += c.x * v.x - c.y * v.y;
+    i1 += c.x * v.y + c.y * v.x;
+  }
+  hxi[0] = make_double2(r0, i0);
+  hxi[1] = make_double2(r1 - r0, i1 - i0);
+  hti[0] = make_double2(r1 - r0, i1 - i0);
+}
+
+// Solution collection on the device (Evaluations.cpp:88-176, first-occurrence
+// uniqueness): one block per instance.
+__global__ void collect_kernel(int n, int K, const double2* __restrict__ roots, uint32_t* __restrict__ flags,
+                               double tol, unsigned long long* __restrict__ counts) {
+  const int inst = blockIdx.x;
+  const size_t base = (size_t)inst * K;
+  unsigned long long c_tr = 0, c_inf = 0, c_ref = 0, c_real = 0, c_uni = 0;
+  for (int j = threadIdx.x; j < K; j += blockDim.x) {
+    uint32_t f = flags[base + j] & ~HCX_FLAG_UNIQUE;
+    c_tr += (f & HCX_FLAG_TRACKED) ? 1 : 0;
+    c_inf += (f & HCX_FLAG_INFINITY) ? 1 : 0;
+    if (f & HCX_FLAG_REFINED) {
+      ++c_ref;
+      c_real += (f & HCX_FLAG_REAL) ? 1 : 0;
+      const double2* xj = roots + (base + j) * n;
+      bool dup = false;
+      for (int i = 0; i < j && !dup; ++i) {
+        if (!(flags[base + i] & HCX_FLAG_REFINED)) continue;
+        const double2* xi = roots + (base + i) * n;
+        bool same = true;
+        for (int v = 0; v < n && same; ++v)
+          same = fabs(xi[v].x - xj[v].x) < tol && fabs(xi[v].y - xj[v].y) < tol;
+        dup = same;
+      }
+      if (!dup) {
+        f |= HCX_FLAG_UNIQUE;
+        ++c_uni;
+      }
+    }
+    flags[base + j] = f;
+  }
+  atomicAdd(&counts[0], c_tr);
+  atomicAdd(&counts[1], c_inf);
+  atomicAdd(&counts[2], c_ref);
+  atomicAdd(&counts[3], c_real);
+  atomicAdd(&counts[4], c_uni);
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// ------------------------------------------------------------ dispatch table
+static const int kNmaxList[] = {4, 8, 16, 32};
+static const int kDegList[] = {2, 3, 4, 6};
+
+static KernelFn pick_kernel(int nmax, int deg, int mode) {
+  switch (nmax * 2 + mode) {
+    case 8: return kernel_n4_m0(deg);
+    case 9: return kernel_n4_m1(deg);
+    case 16: return kernel_n8_m0(deg);
+    case 17: return kernel_n8_m1(deg);
+    case 32: return kernel_n16_m0(deg);
+    case 33: return kernel_n16_m1(deg);
+    case 64: return kernel_n32_m0(deg);
+    case 65: return kernel_n32_m1(deg);
+    default: return nullptr;
+  }
+}
+
+}  // namespace hcx
+
+// ============================================================== problem object
+struct hcx_problem {
+  int device = 0;
+  int n = 0, C = 0, M = 1, K = 0, k = 0;
+  int hxT = 0, htT = 0, hx_vars = 0, ht_vars = 0, deg = 2;
+  hcx::dev::Term* d_hx = nullptr;
+  hcx::dev::Term* d_ht = nullptr;
+  int* d_hx_cnt = nullptr;
+  double2* d_start = nullptr;
+  double2* d_affine = nullptr;
+  double2* d_p0 = nullptr;
+  // launch workspace
+  unsigned int* d_counter = nullptr;
+  int* d_inst_left = nullptr;
+  int inst_cap = 0;
+  hcx_problem_info info{};
+};
+
+namespace hcx {
+namespace {
+
+std::mutex g_mu;
+unsigned long long* g_counts = nullptr;   // device [5]
+unsigned long long* g_stamp = nullptr;    // device [1]
+int g_cu = 0;
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return HCX_ERR_HIP;                                                          \
+    }                                                                              \
+  } while (0)
+
+hcx_status ensure_device_globals() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_counts) return HCX_OK;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  g_cu = prop.multiProcessorCount;
+  HIP_TRY(hipMalloc(&g_counts, 5 * sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc(&g_stamp, sizeof(unsigned long long)));
+  return HCX_OK;
+}
+
+int round_up_idx(const int* list, int cnt, int v) {
+  for (int i = 0; i < cnt; ++i)
+    if (list[i] >= v) return i;
+  return -1;
+}
+
+double f_term(int d) { return 6.0 * d + 4.0; }
+
+// LAPACK Working Note 41 counts for one complex GETRF + GETRS with 1 RHS.
+double f_lu(int n) { return (8.0 / 3.0) * n * n * n + 7.0 * n * n + (7.0 / 3.0) * n; }
+
+hcx_status validate_opts(const hcx_track_opts* o) {
+  if (!o) {
+    set_error("options are NULL");
+    return HCX_ERR_INVALID_ARG;
+  }
+  if (o->max_steps < 0 || o->max_corr < 1 || o->incr_steps < 1 || !(o->dt0 > 0) || !(o->end_zone >= 0) ||
+      !(o->t_tol > 0) || o->refine_iters < 0 || (o->precision != HCX_PREC_FP64 && o->precision != HCX_PREC_MIXED) ||
+      o->lanes_per_path < 0 || o->lanes_per_path > 64) {
+    set_error("invalid tracker options (max_steps=%d max_corr=%d incr=%d precision=%d lanes=%d)", o->max_steps,
+              o->max_corr, o->incr_steps, o->precision, o->lanes_per_path);
+    return HCX_ERR_INVALID_ARG;
+  }
+  return HCX_OK;
+}
+
+Opts to_opts(const hcx_track_opts* o) {
+  Opts r;
+  r.max_steps = o->max_steps;
+  r.max_corr = o->max_corr;
+  r.incr = o->incr_steps;
+  r.refine_iters = o->refine_iters;
+  r.dt0 = o->dt0;
+  r.end_zone = o->end_zone;
+  r.t_tol = o->t_tol;
+  r.dt_min = o->dt_min;
+  r.corr_tol = o->corr_tol;
+  r.inf_tol = o->inf_tol;
+  r.refine_tol = o->refine_tol;
+  r.real_tol = o->real_tol;
+  return r;
+}
+
+// Per-group LDS: x and pivot-row buffers (FP64 capacity) and the coefficient
+// area: FP64 c + dc/dt (mode FP64, or mixed with M > 1 for the FP64
+// re-track), else FP32 c + dc/dt overlaid by FP64 c.
+int lds_group_bytes(int nmax, int C, int mode, int M) {
+  const int per = (mode == HCX_PREC_FP64 || M > 1) ? 32 : 16;
+  const int b = 2 * 16 * (nmax + 1) + (C + 1) * per;
+  return (b + 15) & ~15;
+}
+
+struct Plan {
+  int nmax_idx = 0, deg_idx = 0, mode = 0;
+  size_t lds = 0;
+  KernelFn fn = nullptr;
+};
+
+// Fills the segment-independent part of a Seg from a problem.
+void seg_from_problem(Seg& S, const hcx_problem* p, int G, int mode, int nmax) {
+  std::memset(&S, 0, sizeof(S));
+  S.n = p->n;
+  S.C = p->C;
+  S.M = p->M;
+  S.K = p->K;
+  S.hxT = p->hxT;
+  S.htT = p->htT;
+  S.G = G;
+  S.lds_group = lds_group_bytes(nmax, p->C, mode, p->M);
+  S.hx = p->d_hx;
+  S.ht = p->d_ht;
+  S.hx_cnt = p->d_hx_cnt;
+  S.start = p->d_start;
+  S.start_stride = p->n;
+  S.roots_stride = p->n;
+  S.counter = p->d_counter;
+}
+
+hcx_status plan_launch(const Launch& LN, int mode, int max_n, int max_deg, Plan& pl) {
+  pl.mode = mode;
+  pl.nmax_idx = round_up_idx(kNmaxList, 4, max_n);
+  pl.deg_idx = round_up_idx(kDegList, 4, max_deg);
+  if (pl.nmax_idx < 0 || pl.deg_idx < 0) {
+    set_error("unsupported sizes: n=%d (max %d), degree=%d (max 6)", max_n, kMaxN, max_deg);
+    return HCX_ERR_UNSUPPORTED;
+  }
+  pl.fn = pick_kernel(kNmaxList[pl.nmax_idx], kDegList[pl.deg_idx], mode);
+  if (!pl.fn) {
+    set_error("no kernel instance for n=%d degree=%d mode=%d", max_n, max_deg, mode);
+    return HCX_ERR_UNSUPPORTED;
+  }
+  size_t lds = 0;
+  for (int s = 0; s < LN.nseg; ++s) {
+    const Seg& S = LN.seg[s];
+    const int groups = (64 + S.G - 1) / S.G;
+    lds = std::max(lds, (size_t)groups * (size_t)S.lds_group);
+  }
+  pl.lds = lds;
+  if (lds > 65536) {
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(pl.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+  }
+  return HCX_OK;
+}
+
+// Default lanes per path: n (several paths per wave, G need not divide 64).
+int default_lanes(int n) { return n; }
+
+hcx_status ensure_inst_cap(hcx_problem* p, int I) {
+  if (p->inst_cap >= I) return HCX_OK;
+  if (p->d_inst_left) (void)hipFree(p->d_inst_left);
+  p->d_inst_left = nullptr;
+  p->inst_cap = 0;
+  HIP_TRY(hipMalloc(&p->d_inst_left, sizeof(int) * (size_t)std::max(I, 1)));
+  p->inst_cap = std::max(I, 1);
+  return HCX_OK;
+}
+
+}  // namespace
+}  // namespace hcx
+
+using namespace hcx;
+
+extern "C" {
+
+int hcx_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return c;
+}
+
+hcx_status hcx_problem_create(const hcx_problem_desc* d, hcx_problem** out) {
+  clear_error();
+  if (!d || !out) {
+    set_error("hcx_problem_create: null argument");
+    return HCX_ERR_INVALID_ARG;
+  }
+  const int n = d->num_vars, C = d->num_coeffs, M = d->max_order_t;
+  if (n < 1 || n > kMaxN || C < 1 || M < 1 || d->hx_terms < 1 || d->ht_terms < 1 || d->hx_parts < 3 ||
+      d->ht_parts < 3 || d->hx_parts > 10 || d->ht_parts > 10 || !d->h_hx_idx || !d->h_ht_idx ||
+      d->num_tracks < 1 || !d->h_start_sols || d->num_params < 0 ||
+      (d->num_params > 0 && (!d->h_coeff_affine || !d->h_start_params))) {
+    set_error("hcx_problem_create: invalid descriptor (n=%d C=%d M=%d hx %dx%d ht %dx%d K=%d k=%d)", n, C, M,
+              d->hx_terms, d->hx_parts, d->ht_terms, d->ht_parts, d->num_tracks, d->num_params);
+    return HCX_ERR_INVALID_ARG;
+  }
+  if (hcx_device_count() < 1) {
+    set_error("hcx_problem_create: no HIP device available");
+    return HCX_ERR_NO_DEVICE;
+  }
+  hcx_status st = ensure_device_globals();
+  if (st != HCX_OK) return st;
+
+  const int hxv = d->hx_parts - 2, htv = d->ht_parts - 2;
+  // repack + validate tables: hx [var j][term][eq i], ht [term][eq i]
+  std::vector<dev::Term> hx((size_t)n * d->hx_terms * n), ht((size_t)d->ht_terms * n);
+  std::vector<int> hx_cnt(n, 0);
+  auto pack = [&](const int* rec, int nv, dev::Term& tm) -> bool {
+    tm.s = rec[0];
+    tm.c = rec[1];
+    if (tm.c < 0 || tm.c > C) return false;
+    uint32_t v03 = 0, v47 = 0;
+    for (int q = 0; q < 8; ++q) {
+      int v = q < nv ? rec[2 + q] : n;
+      if (v < 0 || v > n) return false;
+      if (q < 4) v03 |= (uint32_t)v << (8 * q);
+      else v47 |= (uint32_t)v << (8 * (q - 4));
+    }
+    tm.v03 = v03;
+    tm.v47 = v47;
+    return true;
+  };
+  int max_deg = 1;
+  for (int i = 0; i < n; ++i) {
+    for (int tt = 0; tt < d->ht_terms; ++tt) {
+      const int* rec = d->h_ht_idx + ((size_t)i * d->ht_terms + tt) * d->ht_parts;
+      if (!pack(rec, htv, ht[(size_t)tt * n + i])) {
+        set_error("hcx_problem_create: dHdt entry (eq %d, term %d) out of range", i, tt);
+        return HCX_ERR_INVALID_ARG;
+      }
+      if (rec[0] != 0) {
+        int dg = 0;
+        for (int q = 0; q < htv; ++q) dg += rec[2 + q] != n;
+        max_deg = std::max(max_deg, dg);
+      }
+    }
+    for (int j = 0; j < n; ++j) {
+      int real = 0;
+      for (int tt = 0; tt < d->hx_terms; ++tt) {
+        const int* rec = d->h_hx_idx + (((size_t)i * n + j) * d->hx_terms + tt) * d->hx_parts;
+        if (!pack(rec, hxv, hx[((size_t)j * d->hx_terms + tt) * n + i])) {
+          set_error("hcx_problem_create: dHdx entry (eq %d, var %d, term %d) out of range", i, j, tt);
+          return HCX_ERR_INVALID_ARG;
+        }
+        if (rec[0] != 0) {
+          real = tt + 1;
+          int dg = 1;
+          for (int q = 0; q < hxv; ++q) dg += rec[2 + q] != n;
+          max_deg = std::max(max_deg, dg);
+        }
+      }
+      hx_cnt[j] = std::max(hx_cnt[j], real);
+    }
+  }
+  // a padded H term still multiplies ht_vars factors; the kernel reads DEG slots
+  const int deg = std::max(std::max(max_deg, htv), hxv + 1);
+  if (deg > 6) {
+    set_error("hcx_problem_create: terms with %d variable factors are not supported (max 6)", deg);
+    return HCX_ERR_UNSUPPORTED;
+  }
+
+  auto* p = new hcx_problem();
+  (void)hipGetDevice(&p->device);
+  p->n = n;
+  p->C = C;
+  p->M = M;
+  p->K = d->num_tracks;
+  p->k = d->num_params;
+  p->hxT = d->hx_terms;
+  p->htT = d->ht_terms;
+  p->hx_vars = hxv;
+  p->ht_vars = htv;
+  p->deg = std::max(deg, 2);
+  auto fail = [&](hipError_t e, const char* what) {
+    set_error("hcx_problem_create: %s: %s", what, hipGetErrorString(e));
+    hcx_problem_destroy(p);
+    return HCX_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&p->d_hx, hx.size() * sizeof(dev::Term))) != hipSuccess) return fail(e, "hipMalloc hx");
+  if ((e = hipMalloc(&p->d_ht, ht.size() * sizeof(dev::Term))) != hipSuccess) return fail(e, "hipMalloc ht");
+  if ((e = hipMalloc(&p->d_hx_cnt, n * sizeof(int))) != hipSuccess) return fail(e, "hipMalloc hx_cnt");
+  if ((e = hipMalloc(&p->d_start, (size_t)p->K * n * sizeof(double2))) != hipSuccess) return fail(e, "hipMalloc start");
+  if ((e = hipMalloc(&p->d_counter, 64 * sizeof(unsigned int))) != hipSuccess) return fail(e, "hipMalloc counter");
+  if ((e = hipMemcpy(p->d_hx, hx.data(), hx.size() * sizeof(dev::Term), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "copy hx");
+  if ((e = hipMemcpy(p->d_ht, ht.data(), ht.size() * sizeof(dev::Term), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "copy ht");
+  if ((e = hipMemcpy(p->d_hx_cnt, hx_cnt.data(), n * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "copy hx_cnt");
+  if ((e = hipMemcpy(p->d_start, d->h_start_sols, (size_t)p->K * n * sizeof(double2), hipMemcpyHostToDevice)) !=
+      hipSuccess)
+    return fail(e, "copy start");
+  if (p->k > 0) {
+    if ((e = hipMalloc(&p->d_affine, (size_t)C * (1 + p->k) * sizeof(double2))) != hipSuccess)
+      return fail(e, "hipMalloc affine");
+    if ((e = hipMalloc(&p->d_p0, (size_t)p->k * sizeof(double2))) != hipSuccess) return fail(e, "hipMalloc p0");
+    if ((e = hipMemcpy(p->d_affine, d->h_coeff_affine, (size_t)C * (1 + p->k) * sizeof(double2),
+                       hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(e, "copy affine");
+    if ((e = hipMemcpy(p->d_p0, d->h_start_params, (size_t)p->k * sizeof(double2), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+      return fail(e, "copy p0");
+  }
+
+  // algorithmic FLOPs of the reference algorithm (padding terms excluded)
+  hcx_problem_info& in = p->info;
+  in.num_vars = n;
+  in.num_coeffs = C;
+  in.max_order_t = M;
+  in.num_tracks = p->K;
+  in.num_params = p->k;
+  in.hx_terms = p->hxT;
+  in.ht_terms = p->htT;
+  in.hx_vars = hxv;
+  in.ht_vars = htv;
+  double fh = 0, fhx = 0;
+  for (int i = 0; i < n; ++i) {
+    for (int tt = 0; tt < d->ht_terms; ++tt) {
+      const int* rec = d->h_ht_idx + ((size_t)i * d->ht_terms + tt) * d->ht_parts;
+      if (rec[0] == 0) continue;
+      int dg = 0;
+      for (int q = 0; q < htv; ++q) dg += rec[2 + q] != n;
+      fh += f_term(dg);
+    }
+    for (int j = 0; j < n; ++j)
+      for (int tt = 0; tt < d->hx_terms; ++tt) {
+        const int* rec = d->h_hx_idx + (((size_t)i * n + j) * d->hx_terms + tt) * d->hx_parts;
+        if (rec[0] == 0) continue;
+        int dg = 0;
+        for (int q = 0; q < hxv; ++q) dg += rec[2 + q] != n;
+        fhx += f_term(dg);
+      }
+  }
+  in.flops_coef = (double)C * (8.0 * M - 4.0);
+  in.flops_hx = fhx;
+  in.flops_ht = fh;
+  in.flops_h = fh;
+  in.flops_lu = f_lu(n);
+  in.flops_stage = in.flops_coef + fhx + fh + in.flops_lu + 16.0 * n;
+  in.flops_newton = fhx + fh + in.flops_lu + 12.0 * n;
+  *out = p;
+  return HCX_OK;
+}
+
+hcx_status hcx_problem_destroy(hcx_problem* p) {
+  if (!p) return HCX_OK;
+  if (p->d_hx) (void)hipFree(p->d_hx);
+  if (p->d_ht) (void)hipFree(p->d_ht);
+  if (p->d_hx_cnt) (void)hipFree(p->d_hx_cnt);
+  if (p->d_start) (void)hipFree(p->d_start);
+  if (p->d_affine) (void)hipFree(p->d_affine);
+  if (p->d_p0) (void)hipFree(p->d_p0);
+  if (p->d_counter) (void)hipFree(p->d_counter);
+  if (p->d_inst_left) (void)hipFree(p->d_inst_left);
+  delete p;
+  return HCX_OK;
+}
+
+hcx_status hcx_problem_get_info(const hcx_problem* p, hcx_problem_info* info) {
+  clear_error();
+  if (!p || !info) {
+    set_error("hcx_problem_get_info: null argument");
+    return HCX_ERR_INVALID_ARG;
+  }
+  *info = p->info;
+  return HCX_OK;
+}
+
+hcx_status hcx_sync(void* stream) {
+  clear_error();
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  return HCX_OK;
+}
+
+// Shared implementation of hcx_track / hcx_track_multi.
+static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc* batches,
+                             const hcx_result_desc* results, const hcx_track_opts* o, hipStream_t stream,
+                             int persistent, unsigned long long* batch_start) {
+  hcx_status st = validate_opts(o);
+  if (st != HCX_OK) return st;
+  if (nseg < 1 || nseg > kMaxSeg || !probs || !batches || !results) {
+    set_error("track: need 1..%d segments with problems, batches and results", kMaxSeg);
+    return HCX_ERR_INVALID_ARG;
+  }
+  st = ensure_device_globals();
+  if (st != HCX_OK) return st;
+  Launch LN;
+  std::memset(&LN, 0, sizeof(LN));
+  LN.nseg = nseg;
+  LN.persistent = persistent;
+  LN.o = to_opts(o);
+  int max_n = 1, max_deg = 2;
+  for (int s = 0; s < nseg; ++s) {
+    hcx_problem* p = probs[s];
+    if (!p) {
+      set_error("track: segment %d has no problem", s);
+      return HCX_ERR_INVALID_ARG;
+    }
+    for (int q = 0; q < s; ++q)
+      if (probs[q] == p) {
+        set_error("track: problem used by two segments of one launch");
+        return HCX_ERR_INVALID_ARG;
+      }
+    max_n = std::max(max_n, p->n);
+    max_deg = std::max(max_deg, p->deg);
+  }
+  const int nmax_idx = round_up_idx(kNmaxList, 4, max_n);
+  if (nmax_idx < 0) {
+    set_error("track: n=%d exceeds %d", max_n, kMaxN);
+    return HCX_ERR_UNSUPPORTED;
+  }
+  const int nmax = kNmaxList[nmax_idx];
+  long long total_groups_needed = 0;
+  std::vector<long long> seg_groups(nseg);
+  for (int s = 0; s < nseg; ++s) {
+    hcx_problem* p = probs[s];
+    const hcx_batch_desc& b = batches[s];
+    const hcx_result_desc& r = results[s];
+    if (b.num_instances < 0 || !r.d_roots || !r.d_flags) {
+      set_error("track: segment %d: bad batch or result descriptor", s);
+      return HCX_ERR_INVALID_ARG;
+    }
+    int G = o->lanes_per_path > 0 ? o->lanes_per_path : default_lanes(p->n);
+    if (G < p->n) {
+      set_error("track: lanes_per_path=%d is below n=%d", G, p->n);
+      return HCX_ERR_INVALID_ARG;
+    }
+    Seg& S = LN.seg[s];
+    seg_from_problem(S, p, G, o->precision, nmax);
+    S.I = b.num_instances;
+    S.P = (long long)b.num_instances * p->K;
+    if (b.d_coeffs_hx) {
+      if (!b.d_coeffs_ht) {
+        set_error("track: segment %d: coefficient tables need both Hx and Ht", s);
+        return HCX_ERR_INVALID_ARG;
+      }
+      S.chx = reinterpret_cast<const double2*>(b.d_coeffs_hx);
+      S.cht = reinterpret_cast<const double2*>(b.d_coeffs_ht);
+    } else {
+      if (!b.d_target_params || !b.d_coeff_work || p->k < 1 || p->M != 1) {
+        set_error("track: segment %d: needs coefficient tables or target params + affine map + workspace", s);
+        return HCX_ERR_INVALID_ARG;
+      }
+      double2* hx = reinterpret_cast<double2*>(b.d_coeff_work);
+      double2* ht = hx + (size_t)b.num_instances * (p->C + 1) * 2;
+      if (b.num_instances > 0) {
+        const double2* p0 = b.d_start_params ? reinterpret_cast<const double2*>(b.d_start_params) : p->d_p0;
+        const int p0_stride = b.d_start_params ? p->k : 0;
+        dim3 grid((p->C + 1 + 127) / 128, b.num_instances);
+        hipLaunchKernelGGL(p2c_affine_kernel, grid, dim3(128), 0, stream, reinterpret_cast<const double2*>(p->d_affine),
+                           p->C, p->k, p0, p0_stride, reinterpret_cast<const double2*>(b.d_target_params),
+                           b.num_instances, hx, ht);
+      }
+      S.chx = hx;
+      S.cht = ht;
+    }
+    S.roots = reinterpret_cast<double2*>(r.d_roots);
+    S.flags = r.d_flags;
+    S.steps = r.d_steps;
+    S.newton = r.d_newton;
+    S.refine = r.d_refine;
+    S.tfin = r.d_t_final;
+    S.dtfin = r.d_dt_final;
+    S.resid = r.d_residual;
+    S.inst_done = reinterpret_cast<unsigned long long*>(r.d_inst_done);
+    if (r.d_inst_done) {
+      st = ensure_inst_cap(p, b.num_instances);
+      if (st != HCX_OK) return st;
+      S.inst_left = p->d_inst_left;
+    }
+    const int gpw = 64 / G;
+    seg_groups[s] = (S.P + gpw - 1) / gpw;
+    total_groups_needed += seg_groups[s];
+  }
+  Plan pl;
+  st = plan_launch(LN, o->precision, max_n, max_deg, pl);
+  if (st != HCX_OK) return st;
+
+  // grid: persistent -> occupancy-limited, split over segments by work
+  long long blocks;
+  if (persistent) {
+    int per_cu = o->waves_per_cu;
+    if (per_cu <= 0) {
+      int nb = 0;
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(pl.fn), 64, pl.lds));
+      per_cu = std::max(nb, 1);
+    }
+    blocks = std::min<long long>((long long)per_cu * g_cu, std::max<long long>(total_groups_needed, 1));
+    // proportional split by estimated work (paths * n^2 as a cost proxy)
+    std::vector<double> w(nseg);
+    double wsum = 0;
+    for (int s = 0; s < nseg; ++s) {
+      w[s] = (double)LN.seg[s].P * (double)(probs[s]->n * probs[s]->n + probs[s]->C);
+      wsum += w[s];
+    }
+    long long acc = 0;
+    for (int s = 0; s < nseg; ++s) {
+      LN.first_block[s] = (int)acc;
+      long long nb = wsum > 0 ? (long long)std::llround(blocks * w[s] / wsum) : 0;
+      nb = std::min(nb, seg_groups[s]);
+      acc += nb;
+    }
+    LN.first_block[nseg] = (int)acc;
+    if (acc < blocks) blocks = std::max<long long>(acc, 1);
+    if (acc == 0) blocks = 1;
+  } else {
+    long long acc = 0;
+    for (int s = 0; s < nseg; ++s) {
+      LN.first_block[s] = (int)acc;
+      acc += seg_groups[s];
+    }
+    LN.first_block[nseg] = (int)acc;
+    blocks = std::max<long long>(acc, 1);
+  }
+  int max_inst = 1;
+  for (int s = 0; s < nseg; ++s) max_inst = std::max(max_inst, LN.seg[s].I);
+  dim3 ig((max_inst + 255) / 256, nseg);
+  hipLaunchKernelGGL(init_kernel, ig, dim3(256), 0, stream, LN, batch_start);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(pl.fn, dim3((unsigned)blocks), dim3(64), pl.lds, stream, LN);
+  HIP_TRY(hipGetLastError());
+  return HCX_OK;
+}
+
+hcx_status hcx_track(hcx_problem* p, const hcx_batch_desc* b, const hcx_track_opts* o, const hcx_result_desc* r,
+                     void* stream) {
+  clear_error();
+  if (!p || !b || !r) {
+    set_error("hcx_track: null argument");
+    return HCX_ERR_INVALID_ARG;
+  }
+  return track_impl(1, &p, b, r, o, (hipStream_t)stream, 1, reinterpret_cast<unsigned long long*>(r->d_batch_start));
+}
+
+hcx_status hcx_track_multi(int num_segments, hcx_problem* const* problems, const hcx_batch_desc* batches,
+                           const hcx_result_desc* results, const hcx_track_opts* o, void* stream) {
+  clear_error();
+  if (num_segments < 1 || !results) {
+    set_error("hcx_track_multi: no segments");
+    return HCX_ERR_INVALID_ARG;
+  }
+  return track_impl(num_segments, problems, batches, results, o, (hipStream_t)stream, 1,
+                    reinterpret_cast<unsigned long long*>(results[0].d_batch_start));
+}
+
+double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, int max_corr, int incr_steps,
+                            const hcx_complex* d_start_sols, hcx_complex* d_track, const hcx_complex* d_coeffs_hx,
+                            const hcx_complex* d_coeffs_ht, uint8_t* d_converge, uint8_t* d_infinity,
+                            hcx_complex* d_debug) {
+  clear_error();
+  hcx_problem* p = const_cast<hcx_problem*>(pc);
+  if (!p || !d_start_sols || !d_track || !d_coeffs_hx || !d_coeffs_ht || !d_converge || !d_infinity) {
+    set_error("hcx_kernel_HC_Solver: null argument");
+    return -1.0;
+  }
+  hcx_track_opts o;
+  hcx_track_opts_default(&o);
+  o.max_steps = max_steps;
+  o.max_corr = max_corr;
+  o.incr_steps = incr_steps;
+  o.dt_min = 0.0;
+  o.refine_iters = 0;
+  if (validate_opts(&o) != HCX_OK) return -1.0;
+  if (ensure_device_globals() != HCX_OK) return -1.0;
+  hipStream_t s = (hipStream_t)stream;
+  Launch LN;
+  std::memset(&LN, 0, sizeof(LN));
+  LN.nseg = 1;
+  LN.persistent = 0;
+  LN.o = to_opts(&o);
+  const int G = default_lanes(p->n);
+  const int nmax_idx = round_up_idx(kNmaxList, 4, p->n);
+  Seg& S = LN.seg[0];
+  seg_from_problem(S, p, G, HCX_PREC_FP64, kNmaxList[nmax_idx]);
+  S.start = reinterpret_cast<const double2*>(d_start_sols);
+  S.start_stride = p->n + 1;
+  S.roots = reinterpret_cast<double2*>(d_track);
+  S.roots_stride = p->n + 1;
+  S.I = 1;
+  S.P = p->K;
+  S.chx = reinterpret_cast<const double2*>(d_coeffs_hx);
+  S.cht = reinterpret_cast<const double2*>(d_coeffs_ht);
+  // flags are needed by the kernel; reuse the problem's scratch via a small buffer
+  static thread_local uint32_t* flags_buf = nullptr;
+  static thread_local int flags_cap = 0;
+  if (flags_cap < p->K) {
+    if (flags_buf) (void)hipFree(flags_buf);
+    if (hipMalloc(&flags_buf, sizeof(uint32_t) * p->K) != hipSuccess) {
+      set_error("hcx_kernel_HC_Solver: hipMalloc flags failed");
+      flags_cap = 0;
+      flags_buf = nullptr;
+      return -1.0;
+    }
+    flags_cap = p->K;
+  }
+  S.flags = flags_buf;
+  S.conv8 = d_converge;
+  S.inf8 = d_infinity;
+  S.debug = reinterpret_cast<double2*>(d_debug);
+  const int gpw = 64 / G;
+  LN.first_block[0] = 0;
+  LN.first_block[1] = (p->K + gpw - 1) / gpw;
+  Plan pl;
+  if (plan_launch(LN, HCX_PREC_FP64, p->n, p->deg, pl) != HCX_OK) return -1.0;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    set_error("hcx_kernel_HC_Solver: hipEventCreate failed");
+    return -1.0;
+  }
+  (void)hipEventRecord(e0, s);
+  hipLaunchKernelGGL(pl.fn, dim3(LN.first_block[1]), dim3(64), pl.lds, s, LN);
+  hipError_t le = hipGetLastError();
+  (void)hipEventRecord(e1, s);
+  hipError_t se = hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (le != hipSuccess || se != hipSuccess) {
+    set_error("hcx_kernel_HC_Solver: launch failed: %s", hipGetErrorString(le != hipSuccess ? le : se));
+    return -1.0;
+  }
+  return ms * 1e-3;
+}
+
+// Reference-style per-instance launch with the generic result layout (no
+// persistence, no refill): used as the plain per-problem baseline.
+hcx_status hcx_track_static(hcx_problem* p, const hcx_batch_desc* b, const hcx_track_opts* o,
+                            const hcx_result_desc* r, void* stream) {
+  clear_error();
+  if (!p || !b || !r) {
+    set_error("hcx_track_static: null argument");
+    return HCX_ERR_INVALID_ARG;
+  }
+  return track_impl(1, &p, b, r, o, (hipStream_t)stream, 0, reinterpret_cast<unsigned long long*>(r->d_batch_start));
+}
+
+hcx_status hcx_collect(int n, int num_instances, int num_tracks, const hcx_complex* d_roots, uint32_t* d_flags,
+                       double unique_tol, hcx_counts* out, void* stream) {
+  clear_error();
+  if (n < 1 || num_instances < 0 || num_tracks < 0 || !d_roots || !d_flags || !out) {
+    set_error("hcx_collect: invalid arguments");
+    return HCX_ERR_INVALID_ARG;
+  }
+  hcx_status st = ensure_device_globals();
+  if (st != HCX_OK) return st;
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, g_counts, 5);
+  if (num_instances > 0)
+    hipLaunchKernelGGL(collect_kernel, dim3(num_instances), dim3(256), 0, s, n, num_tracks,
+                       reinterpret_cast<const double2*>(d_roots), d_flags, unique_tol, g_counts);
+  HIP_TRY(hipGetLastError());
+  unsigned long long h[5];
+  HIP_TRY(hipMemcpyAsync(h, g_counts, sizeof(h), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  out->tracked = (int64_t)h[0];
+  out->infinity = (int64_t)h[1];
+  out->refined = (int64_t)h[2];
+  out->real = (int64_t)h[3];
+  out->unique = (int64_t)h[4];
+  return HCX_OK;
+}
+
+}  // extern "C"
