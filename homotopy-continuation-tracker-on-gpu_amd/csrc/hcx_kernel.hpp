@@ -16,21 +16,9 @@ using dev::cmk;
 constexpr int kMaxSeg = 8;
 constexpr int kMaxN = 32;
 
-struct Seg {
-  int n, C, M, K;
-  int hxT, htT;
-  int G;
-  int lds_group;               // bytes of one group's LDS region
-  const dev::Term* hx;         // [n var][hxT][n eq]
-  const dev::Term* ht;         // [htT][n eq]
-  const int* hx_cnt;           // [n] real terms per Jacobian column (max over equations)
-  const double2* start;        // [K][start_stride]
-  int start_stride;
-  int roots_stride;
-  int I;
-  long long P;
-  const double2* chx;          // [I][C+1][M+1]
-  const double2* cht;          // [I][C+1][M]
+// Output pointers of a segment (cold): kept in device memory and read only
+// when a path finishes, so they do not occupy scalar registers in the loop.
+struct SegOut {
   double2* roots;              // [P][roots_stride]
   uint32_t* flags;
   int* steps;
@@ -44,12 +32,33 @@ struct Seg {
   double2* debug;
   unsigned long long* inst_done;
   int* inst_left;
+  int roots_stride;
+};
+
+// Hot, wave-uniform description of one segment (kernel arguments).
+struct Seg {
+  int n, C, M, K;
+  int hxT, htT;
+  int G;
+  int lds_group;               // bytes of one group's LDS region
+  int start_stride;
+  int I;
+  long long P;
+  const dev::Term* hx;         // [n var][hxT][n eq]
+  const dev::Term* ht;         // [htT][n eq]
+  const int* hx_cnt;           // [n] real terms per Jacobian column (max over equations)
+  const double2* start;        // [K][start_stride]
+  const double2* chx;          // [I][C+1][M+1]
+  const double2* cht;          // [I][C+1][M]
   unsigned int* counter;
+  const SegOut* out;           // device copy written by init_kernel
 };
 
 struct Opts {
   int max_steps, max_corr, incr, refine_iters;
-  double dt0, end_zone, t_tol, dt_min, corr_tol, inf_tol, refine_tol, real_tol;
+  double dt0, t_tol, dt_min, corr_tol, inf_tol, refine_tol, real_tol;
+  double ez_enter;   // enter the end zone when |1 - t| <= end_zone + 1e-7
+  double ez_land;    // before the end zone, steps land on 1 - end_zone
 };
 
 struct Launch {
@@ -323,9 +332,11 @@ __device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, cons
 // in the end zone, FP64 refinement; a path that ends without a refined root
 // is re-tracked from t = 0 entirely in FP64 by the same group (escalation).
 template <int NMAX, int DEG, int MODE>
-__device__ void run_segment(const Launch& LN, int si, unsigned char* smem, int lane) {
-  const Seg& S = LN.seg[si];
-  const Opts& o = LN.o;
+__device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned char* smem, int lane) {
+  // local copies: the fields are wave-uniform and live in SGPRs (scalar
+  // kernarg loads) instead of being re-read through a generic pointer
+  const Seg S = LN.seg[si];
+  const Opts o = LN.o;
   const Group g = make_group(S.G, lane);
   const int n = S.n;
   const bool row = g.in_grp && g.gl < n;
@@ -409,11 +420,11 @@ __device__ void run_segment(const Launch& LN, int si, unsigned char* smem, int l
     if (__any(stepping)) {
       // end zone and step clamp (:121-131)
       if (stepping) {
-        if (!ez && fabs(1.0 - t) <= o.end_zone + 1e-7) ez = true;
+        if (!ez && fabs(1.0 - t) <= o.ez_enter) ez = true;
         if (ez) {
           if (dt > fabs(1.0 - t)) dt = fabs(1.0 - t);
-        } else if (dt > fabs(1.0 - o.end_zone - t)) {
-          dt = fabs(1.0 - o.end_zone - t);
+        } else if (dt > fabs(o.ez_land - t)) {
+          dt = fabs(o.ez_land - t);
         }
       }
       const double ts = t;
@@ -539,22 +550,25 @@ __device__ void run_segment(const Launch& LN, int si, unsigned char* smem, int l
           fl = HCX_FLAG_ESCALATED;
           start_path(true);
         } else {
-          if (row) S.roots[(size_t)path * S.roots_stride + g.gl] = make_double2(x.re, x.im);
+          const SegOut* O = S.out;
+          double2* roots = O->roots;
+          const int rstride = O->roots_stride;
+          if (row) roots[(size_t)path * rstride + g.gl] = make_double2(x.re, x.im);
           if (g.gl == 0) {
-            if (S.roots_stride > n) S.roots[(size_t)path * S.roots_stride + n] = make_double2(1.0, 0.0);
-            S.flags[path] = fl;
-            if (S.steps) S.steps[path] = nsteps;
-            if (S.newton) S.newton[path] = nnewt;
-            if (S.refine) S.refine[path] = nref;
-            if (S.tfin) S.tfin[path] = t;
-            if (S.dtfin) S.dtfin[path] = dt;
-            if (S.resid) S.resid[path] = resid;
-            if (S.conv8) S.conv8[path] = (fl & HCX_FLAG_TRACKED) ? 1 : 0;
-            if (S.inf8) S.inf8[path] = (fl & HCX_FLAG_INFINITY) ? 1 : 0;
-            if (S.debug) S.debug[path] = (fl & HCX_FLAG_TRACKED) ? make_double2(1.0, 0.0) : make_double2(t, dt);
-            if (S.inst_left) {
-              const int left = atomicSub(&S.inst_left[inst], 1);
-              if (left == 1 && S.inst_done) S.inst_done[inst] = wall_clock64();
+            if (rstride > n) roots[(size_t)path * rstride + n] = make_double2(1.0, 0.0);
+            O->flags[path] = fl;
+            if (O->steps) O->steps[path] = nsteps;
+            if (O->newton) O->newton[path] = nnewt;
+            if (O->refine) O->refine[path] = nref;
+            if (O->tfin) O->tfin[path] = t;
+            if (O->dtfin) O->dtfin[path] = dt;
+            if (O->resid) O->resid[path] = resid;
+            if (O->conv8) O->conv8[path] = (fl & HCX_FLAG_TRACKED) ? 1 : 0;
+            if (O->inf8) O->inf8[path] = (fl & HCX_FLAG_INFINITY) ? 1 : 0;
+            if (O->debug) O->debug[path] = (fl & HCX_FLAG_TRACKED) ? make_double2(1.0, 0.0) : make_double2(t, dt);
+            if (O->inst_left) {
+              const int left = atomicSub(&O->inst_left[inst], 1);
+              if (left == 1 && O->inst_done) O->inst_done[inst] = wall_clock64();
             }
           }
           status = 0;

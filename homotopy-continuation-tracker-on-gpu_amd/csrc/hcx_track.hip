@@ -35,14 +35,23 @@ namespace hcx {
 using dev::cx;
 using dev::cmk;
 
-// Counters of a launch: path queues, per-instance remaining paths, start stamp.
-__global__ void init_kernel(const Launch LN, unsigned long long* batch_start) {
+struct OutPack {
+  SegOut out[kMaxSeg];
+};
+
+// Launch set-up on the device (graph-safe, no memset): path queues,
+// per-instance remaining-path counters, output descriptors, start stamp.
+__global__ void init_kernel(const Launch LN, const OutPack OP, unsigned long long* batch_start) {
   const int s = blockIdx.y;
   if (s >= LN.nseg) return;
   const Seg& S = LN.seg[s];
+  const SegOut& O = OP.out[s];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *S.counter = 0u;
-  if (S.inst_left && i < S.I) S.inst_left[i] = S.K;
+  if (i == 0) {
+    *S.counter = 0u;
+    *const_cast<SegOut*>(S.out) = O;
+  }
+  if (O.inst_left && i < S.I) O.inst_left[i] = S.K;
   if (batch_start && s == 0 && i == 0) *batch_start = wall_clock64();
 }
 
@@ -156,6 +165,7 @@ struct hcx_problem {
   double2* d_p0 = nullptr;
   // launch workspace
   unsigned int* d_counter = nullptr;
+  hcx::SegOut* d_out = nullptr;
   int* d_inst_left = nullptr;
   int inst_cap = 0;
   hcx_problem_info info{};
@@ -219,12 +229,13 @@ hcx_status validate_opts(const hcx_track_opts* o) {
 
 Opts to_opts(const hcx_track_opts* o) {
   Opts r;
+  r.ez_enter = o->end_zone + 1e-7;
+  r.ez_land = 1.0 - o->end_zone;
   r.max_steps = o->max_steps;
   r.max_corr = o->max_corr;
   r.incr = o->incr_steps;
   r.refine_iters = o->refine_iters;
   r.dt0 = o->dt0;
-  r.end_zone = o->end_zone;
   r.t_tol = o->t_tol;
   r.dt_min = o->dt_min;
   r.corr_tol = o->corr_tol;
@@ -250,8 +261,11 @@ struct Plan {
 };
 
 // Fills the segment-independent part of a Seg from a problem.
-void seg_from_problem(Seg& S, const hcx_problem* p, int G, int mode, int nmax) {
+void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, int nmax) {
   std::memset(&S, 0, sizeof(S));
+  std::memset(&O, 0, sizeof(O));
+  S.out = p->d_out;
+  O.roots_stride = p->n;
   S.n = p->n;
   S.C = p->C;
   S.M = p->M;
@@ -265,7 +279,6 @@ void seg_from_problem(Seg& S, const hcx_problem* p, int G, int mode, int nmax) {
   S.hx_cnt = p->d_hx_cnt;
   S.start = p->d_start;
   S.start_stride = p->n;
-  S.roots_stride = p->n;
   S.counter = p->d_counter;
 }
 
@@ -428,6 +441,7 @@ hcx_status hcx_problem_create(const hcx_problem_desc* d, hcx_problem** out) {
   if ((e = hipMalloc(&p->d_hx_cnt, n * sizeof(int))) != hipSuccess) return fail(e, "hipMalloc hx_cnt");
   if ((e = hipMalloc(&p->d_start, (size_t)p->K * n * sizeof(double2))) != hipSuccess) return fail(e, "hipMalloc start");
   if ((e = hipMalloc(&p->d_counter, 64 * sizeof(unsigned int))) != hipSuccess) return fail(e, "hipMalloc counter");
+  if ((e = hipMalloc(&p->d_out, sizeof(SegOut))) != hipSuccess) return fail(e, "hipMalloc out");
   if ((e = hipMemcpy(p->d_hx, hx.data(), hx.size() * sizeof(dev::Term), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(e, "copy hx");
   if ((e = hipMemcpy(p->d_ht, ht.data(), ht.size() * sizeof(dev::Term), hipMemcpyHostToDevice)) != hipSuccess)
@@ -498,6 +512,7 @@ hcx_status hcx_problem_destroy(hcx_problem* p) {
   if (p->d_affine) (void)hipFree(p->d_affine);
   if (p->d_p0) (void)hipFree(p->d_p0);
   if (p->d_counter) (void)hipFree(p->d_counter);
+  if (p->d_out) (void)hipFree(p->d_out);
   if (p->d_inst_left) (void)hipFree(p->d_inst_left);
   delete p;
   return HCX_OK;
@@ -533,6 +548,8 @@ static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batc
   if (st != HCX_OK) return st;
   Launch LN;
   std::memset(&LN, 0, sizeof(LN));
+  OutPack OP;
+  std::memset(&OP, 0, sizeof(OP));
   LN.nseg = nseg;
   LN.persistent = persistent;
   LN.o = to_opts(o);
@@ -573,7 +590,8 @@ static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batc
       return HCX_ERR_INVALID_ARG;
     }
     Seg& S = LN.seg[s];
-    seg_from_problem(S, p, G, o->precision, nmax);
+    SegOut& O = OP.out[s];
+    seg_from_problem(S, O, p, G, o->precision, nmax);
     S.I = b.num_instances;
     S.P = (long long)b.num_instances * p->K;
     if (b.d_coeffs_hx) {
@@ -601,19 +619,19 @@ static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batc
       S.chx = hx;
       S.cht = ht;
     }
-    S.roots = reinterpret_cast<double2*>(r.d_roots);
-    S.flags = r.d_flags;
-    S.steps = r.d_steps;
-    S.newton = r.d_newton;
-    S.refine = r.d_refine;
-    S.tfin = r.d_t_final;
-    S.dtfin = r.d_dt_final;
-    S.resid = r.d_residual;
-    S.inst_done = reinterpret_cast<unsigned long long*>(r.d_inst_done);
+    O.roots = reinterpret_cast<double2*>(r.d_roots);
+    O.flags = r.d_flags;
+    O.steps = r.d_steps;
+    O.newton = r.d_newton;
+    O.refine = r.d_refine;
+    O.tfin = r.d_t_final;
+    O.dtfin = r.d_dt_final;
+    O.resid = r.d_residual;
+    O.inst_done = reinterpret_cast<unsigned long long*>(r.d_inst_done);
     if (r.d_inst_done) {
       st = ensure_inst_cap(p, b.num_instances);
       if (st != HCX_OK) return st;
-      S.inst_left = p->d_inst_left;
+      O.inst_left = p->d_inst_left;
     }
     const int gpw = 64 / G;
     seg_groups[s] = (S.P + gpw - 1) / gpw;
@@ -662,7 +680,7 @@ static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batc
   int max_inst = 1;
   for (int s = 0; s < nseg; ++s) max_inst = std::max(max_inst, LN.seg[s].I);
   dim3 ig((max_inst + 255) / 256, nseg);
-  hipLaunchKernelGGL(init_kernel, ig, dim3(256), 0, stream, LN, batch_start);
+  hipLaunchKernelGGL(init_kernel, ig, dim3(256), 0, stream, LN, OP, batch_start);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(pl.fn, dim3((unsigned)blocks), dim3(64), pl.lds, stream, LN);
   HIP_TRY(hipGetLastError());
@@ -712,17 +730,20 @@ double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, 
   hipStream_t s = (hipStream_t)stream;
   Launch LN;
   std::memset(&LN, 0, sizeof(LN));
+  OutPack OP;
+  std::memset(&OP, 0, sizeof(OP));
   LN.nseg = 1;
   LN.persistent = 0;
   LN.o = to_opts(&o);
   const int G = default_lanes(p->n);
   const int nmax_idx = round_up_idx(kNmaxList, 4, p->n);
   Seg& S = LN.seg[0];
-  seg_from_problem(S, p, G, HCX_PREC_FP64, kNmaxList[nmax_idx]);
+  SegOut& O = OP.out[0];
+  seg_from_problem(S, O, p, G, HCX_PREC_FP64, kNmaxList[nmax_idx]);
   S.start = reinterpret_cast<const double2*>(d_start_sols);
   S.start_stride = p->n + 1;
-  S.roots = reinterpret_cast<double2*>(d_track);
-  S.roots_stride = p->n + 1;
+  O.roots = reinterpret_cast<double2*>(d_track);
+  O.roots_stride = p->n + 1;
   S.I = 1;
   S.P = p->K;
   S.chx = reinterpret_cast<const double2*>(d_coeffs_hx);
@@ -740,10 +761,10 @@ double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, 
     }
     flags_cap = p->K;
   }
-  S.flags = flags_buf;
-  S.conv8 = d_converge;
-  S.inf8 = d_infinity;
-  S.debug = reinterpret_cast<double2*>(d_debug);
+  O.flags = flags_buf;
+  O.conv8 = d_converge;
+  O.inf8 = d_infinity;
+  O.debug = reinterpret_cast<double2*>(d_debug);
   const int gpw = 64 / G;
   LN.first_block[0] = 0;
   LN.first_block[1] = (p->K + gpw - 1) / gpw;
@@ -754,6 +775,7 @@ double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, 
     set_error("hcx_kernel_HC_Solver: hipEventCreate failed");
     return -1.0;
   }
+  hipLaunchKernelGGL(init_kernel, dim3(1, 1), dim3(64), 0, s, LN, OP, (unsigned long long*)nullptr);
   (void)hipEventRecord(e0, s);
   hipLaunchKernelGGL(pl.fn, dim3(LN.first_block[1]), dim3(64), pl.lds, s, LN);
   hipError_t le = hipGetLastError();
