@@ -33,6 +33,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic():
+    """HBM bytes per launch of the track kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "track_kernel_traffic_n6_param_homotopy_1024x64.md")
+    try:
+        m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def percentile(a, q):
     import numpy as np
     return float(np.percentile(np.asarray(a, dtype=float), q)) if len(a) else float("nan")
@@ -227,7 +239,8 @@ def main():
         "speedup_vs_fp64_per_problem": (value / fp64["value"]) if fp64 else None,
         "parity_mixed_vs_fp64": parity,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_VALU_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_VALU_TFLOPS, "traffic": profiled_traffic(),
+                     "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
                      "kernel_ms": kernel_ms, "algorithmic_flop": flops,
                      "note": "LAWN-41 FLOPs of the reference algorithm from per-path device step/Newton "
                              "counters; peak = vector FP64 (= non-packed FP32 rate)"},
