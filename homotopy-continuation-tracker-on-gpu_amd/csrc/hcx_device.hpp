@@ -73,9 +73,27 @@ __device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) {
     return cmk<T>((a.re * r + a.im) * inv, (a.im * r - a.re) * inv);
   }
 }
+// Reciprocals from the hardware estimate: v_rcp_f32 (1 ulp) for FP32;
+// v_rcp_f64 refined by two Newton-Raphson steps r <- r + r (1 - x r) for FP64.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+// Complex reciprocal 1/b by Smith's scaling (as cdiv) with the fast real reciprocal.
 template <class T>
 __device__ __forceinline__ cx<T> crecip(cx<T> b) {
-  return cdiv<T>(cmk<T>(T(1), T(0)), b);
+  if (fabs(b.re) >= fabs(b.im)) {
+    const T r = b.im * frcp(b.re);
+    const T inv = frcp(b.re + b.im * r);
+    return cmk<T>(inv, -r * inv);
+  } else {
+    const T r = b.re * frcp(b.im);
+    const T inv = frcp(b.re * r + b.im);
+    return cmk<T>(r * inv, -inv);
+  }
 }
 
 template <class T, class U>

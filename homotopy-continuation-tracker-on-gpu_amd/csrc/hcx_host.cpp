@@ -166,6 +166,7 @@ void hcx_track_opts_default(hcx_track_opts* o) {
   o->precision = HCX_PREC_FP64;
   o->lanes_per_path = 0;
   o->waves_per_cu = 0;
+  o->specialize = 1;
 }
 
 hcx_status hcx_random_cn(uint64_t seed, int64_t count, hcx_complex* out) {

@@ -1,0 +1,15 @@
+// Generic tracker kernel instances: NMAX = 32 (any n <= 32, any G >= n), mixed FP32/FP64 mode.
+#include "hcx_kernel.hpp"
+
+namespace hcx {
+
+KernelFn kernel_g32_m1(int deg) {
+  switch (deg) {
+    case 2: return track_kernel<32, 2, 1, false>;
+    case 3: return track_kernel<32, 3, 1, false>;
+    case 6: return track_kernel<32, 6, 1, false>;
+    default: return nullptr;
+  }
+}
+
+}  // namespace hcx

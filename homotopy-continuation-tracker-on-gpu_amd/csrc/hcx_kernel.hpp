@@ -125,6 +125,18 @@ __device__ __forceinline__ void stage_coeffs(const Seg& S, const Group& g, bool 
   const int C1 = S.C + 1, M = S.M;
   const double2* hx = S.chx + (size_t)inst * C1 * (M + 1);
   const double2* ht = S.cht + (size_t)inst * C1 * M;
+  if (M == 1) {
+    // c(t) = c0 + t c1, dc/dt = Ht[j][0]
+    for (int j = g.gl; j < C1; j += g.G) {
+      const double2 a = hx[2 * (size_t)j], b = hx[2 * (size_t)j + 1];
+      sc[j] = cmk<T>((T)(a.x + t * b.x), (T)(a.y + t * b.y));
+      if (with_dot) {
+        const double2 d = ht[j];
+        sd[j] = cmk<T>((T)d.x, (T)d.y);
+      }
+    }
+    return;
+  }
   for (int j = g.gl; j < C1; j += g.G) {
     double2 a = hx[(size_t)j * (M + 1) + M];
     double cr = a.x, ci = a.y;
@@ -331,11 +343,17 @@ __device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, cons
 // predictor and Jacobian, FP32 corrector before the end zone, FP64 corrector
 // in the end zone, FP64 refinement; a path that ends without a refined root
 // is re-tracked from t = 0 entirely in FP64 by the same group (escalation).
-template <int NMAX, int DEG, int MODE>
+template <int NMAX, int DEG, int MODE, bool EXACT>
 __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned char* smem, int lane) {
   // local copies: the fields are wave-uniform and live in SGPRs (scalar
   // kernarg loads) instead of being re-read through a generic pointer
-  const Seg S = LN.seg[si];
+  Seg S = LN.seg[si];
+  if (EXACT) {
+    // size-specialised instance: n = G = NMAX are compile-time constants,
+    // so every loop bound and the lane-group geometry fold away
+    S.n = NMAX;
+    S.G = NMAX;
+  }
   const Opts o = LN.o;
   const Group g = make_group(S.G, lane);
   const int n = S.n;
@@ -444,7 +462,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
         acc = esc ? a64 : a32;
       }
       if (stepping) {
-        const double f = dt / 6.0;
+        const double f = dt * (1.0 / 6.0);
         x = cmk<double>(xl.re + f * acc.re, xl.im + f * acc.im);
         t = ts + dt;
       }
@@ -579,14 +597,14 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
   }
 }
 
-template <int NMAX, int DEG, int MODE>
+template <int NMAX, int DEG, int MODE, bool EXACT>
 __global__ __launch_bounds__(64) void track_kernel(const Launch LN) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x;
   int s = 0;
   while (s + 1 < LN.nseg && (int)blockIdx.x >= LN.first_block[s + 1]) ++s;
   for (int tried = 0; tried < LN.nseg; ++tried) {
-    run_segment<NMAX, DEG, MODE>(LN, s, smem, lane);
+    run_segment<NMAX, DEG, MODE, EXACT>(LN, s, smem, lane);
     if (!LN.persistent) break;
     s = s + 1 == LN.nseg ? 0 : s + 1;
   }
@@ -595,13 +613,28 @@ __global__ __launch_bounds__(64) void track_kernel(const Launch LN) {
 using KernelFn = void (*)(const Launch);
 
 // Kernel instances live in separate translation units (compile in parallel).
-KernelFn kernel_n4_m0(int deg);
-KernelFn kernel_n8_m0(int deg);
-KernelFn kernel_n16_m0(int deg);
-KernelFn kernel_n32_m0(int deg);
-KernelFn kernel_n4_m1(int deg);
-KernelFn kernel_n8_m1(int deg);
-KernelFn kernel_n16_m1(int deg);
-KernelFn kernel_n32_m1(int deg);
+// Generic instances: any n <= NMAX and any lanes-per-path G >= n.
+KernelFn kernel_g8_m0(int deg);
+KernelFn kernel_g8_m1(int deg);
+KernelFn kernel_g16_m0(int deg);
+KernelFn kernel_g16_m1(int deg);
+KernelFn kernel_g32_m0(int deg);
+KernelFn kernel_g32_m1(int deg);
+// Size-specialised instances (n = G = N) for the problem sizes of the
+// benchmark workloads, DEG 2 and 3.
+KernelFn kernel_x3_m0(int deg);
+KernelFn kernel_x3_m1(int deg);
+KernelFn kernel_x5_m0(int deg);
+KernelFn kernel_x5_m1(int deg);
+KernelFn kernel_x6_m0(int deg);
+KernelFn kernel_x6_m1(int deg);
+KernelFn kernel_x9_m0(int deg);
+KernelFn kernel_x9_m1(int deg);
+KernelFn kernel_x12_m0(int deg);
+KernelFn kernel_x12_m1(int deg);
+KernelFn kernel_x16_m0(int deg);
+KernelFn kernel_x16_m1(int deg);
+KernelFn kernel_x30_m0(int deg);
+KernelFn kernel_x30_m1(int deg);
 
 }  // namespace hcx

@@ -133,19 +133,29 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
 }
 
 // ------------------------------------------------------------ dispatch table
-static const int kNmaxList[] = {4, 8, 16, 32};
-static const int kDegList[] = {2, 3, 4, 6};
+static const int kNmaxList[] = {8, 16, 32};
+static const int kDegList[] = {2, 3, 6};
 
 static KernelFn pick_kernel(int nmax, int deg, int mode) {
-  switch (nmax * 2 + mode) {
-    case 8: return kernel_n4_m0(deg);
-    case 9: return kernel_n4_m1(deg);
-    case 16: return kernel_n8_m0(deg);
-    case 17: return kernel_n8_m1(deg);
-    case 32: return kernel_n16_m0(deg);
-    case 33: return kernel_n16_m1(deg);
-    case 64: return kernel_n32_m0(deg);
-    case 65: return kernel_n32_m1(deg);
+  switch (nmax) {
+    case 8: return mode == 0 ? kernel_g8_m0(deg) : kernel_g8_m1(deg);
+    case 16: return mode == 0 ? kernel_g16_m0(deg) : kernel_g16_m1(deg);
+    case 32: return mode == 0 ? kernel_g32_m0(deg) : kernel_g32_m1(deg);
+    default: return nullptr;
+  }
+}
+
+static KernelFn pick_exact(int n, int deg, int mode) {
+  if (deg > 3) return nullptr;
+  deg = deg < 2 ? 2 : deg;
+  switch (n) {
+    case 3: return mode == 0 ? kernel_x3_m0(deg) : kernel_x3_m1(deg);
+    case 5: return mode == 0 ? kernel_x5_m0(deg) : kernel_x5_m1(deg);
+    case 6: return mode == 0 ? kernel_x6_m0(deg) : kernel_x6_m1(deg);
+    case 9: return mode == 0 ? kernel_x9_m0(deg) : kernel_x9_m1(deg);
+    case 12: return mode == 0 ? kernel_x12_m0(deg) : kernel_x12_m1(deg);
+    case 16: return mode == 0 ? kernel_x16_m0(deg) : kernel_x16_m1(deg);
+    case 30: return mode == 0 ? kernel_x30_m0(deg) : kernel_x30_m1(deg);
     default: return nullptr;
   }
 }
@@ -219,7 +229,7 @@ hcx_status validate_opts(const hcx_track_opts* o) {
   }
   if (o->max_steps < 0 || o->max_corr < 1 || o->incr_steps < 1 || !(o->dt0 > 0) || !(o->end_zone >= 0) ||
       !(o->t_tol > 0) || o->refine_iters < 0 || (o->precision != HCX_PREC_FP64 && o->precision != HCX_PREC_MIXED) ||
-      o->lanes_per_path < 0 || o->lanes_per_path > 64) {
+      o->lanes_per_path < 0 || o->lanes_per_path > 64 || o->specialize < 0 || o->specialize > 1) {
     set_error("invalid tracker options (max_steps=%d max_corr=%d incr=%d precision=%d lanes=%d)", o->max_steps,
               o->max_corr, o->incr_steps, o->precision, o->lanes_per_path);
     return HCX_ERR_INVALID_ARG;
@@ -248,17 +258,59 @@ Opts to_opts(const hcx_track_opts* o) {
 // Per-group LDS: x and pivot-row buffers (FP64 capacity) and the coefficient
 // area: FP64 c + dc/dt (mode FP64, or mixed with M > 1 for the FP64
 // re-track), else FP32 c + dc/dt overlaid by FP64 c.
+// The region stride is padded to 16 mod 256 bytes: the groups of a wave read
+// the same offsets of their regions together (pivot-row broadcasts, x and
+// coefficient gathers), and this stride spreads them over distinct LDS banks
+// (64 banks x 4 bytes) for 8- and 16-byte accesses.
 int lds_group_bytes(int nmax, int C, int mode, int M) {
   const int per = (mode == HCX_PREC_FP64 || M > 1) ? 32 : 16;
-  const int b = 2 * 16 * (nmax + 1) + (C + 1) * per;
-  return (b + 15) & ~15;
+  int b = 2 * 16 * (nmax + 1) + (C + 1) * per;
+  b = (b + 15) & ~15;
+  return ((b - 16 + 255) / 256) * 256 + 16;
 }
 
 struct Plan {
-  int nmax_idx = 0, deg_idx = 0, mode = 0;
+  int nmax = 0, mode = 0;
+  bool exact = false;
   size_t lds = 0;
   KernelFn fn = nullptr;
 };
+
+// Kernel instance for a launch: the size-specialised one when every segment
+// has the same n, runs one lane per row (G = n) and fits DEG <= 3; otherwise
+// the generic instance with the smallest NMAX >= max n.
+hcx_status choose_kernel(int nseg, const int* ns, const int* Gs, int max_deg, int mode, int specialize, Plan& pl) {
+  pl.mode = mode;
+  int max_n = 1;
+  bool same = true;
+  for (int s = 0; s < nseg; ++s) {
+    max_n = std::max(max_n, ns[s]);
+    same = same && ns[s] == ns[0] && Gs[s] == ns[s];
+  }
+  if (specialize && same) {
+    KernelFn fn = pick_exact(ns[0], max_deg, mode);
+    if (fn) {
+      pl.fn = fn;
+      pl.nmax = ns[0];
+      pl.exact = true;
+      return HCX_OK;
+    }
+  }
+  const int ni = round_up_idx(kNmaxList, 3, max_n);
+  const int di = round_up_idx(kDegList, 3, max_deg);
+  if (ni < 0 || di < 0) {
+    set_error("unsupported sizes: n=%d (max %d), degree=%d (max 6)", max_n, kMaxN, max_deg);
+    return HCX_ERR_UNSUPPORTED;
+  }
+  pl.fn = pick_kernel(kNmaxList[ni], kDegList[di], mode);
+  pl.nmax = kNmaxList[ni];
+  pl.exact = false;
+  if (!pl.fn) {
+    set_error("no kernel instance for n=%d degree=%d mode=%d", max_n, max_deg, mode);
+    return HCX_ERR_UNSUPPORTED;
+  }
+  return HCX_OK;
+}
 
 // Fills the segment-independent part of a Seg from a problem.
 void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, int nmax) {
@@ -282,19 +334,7 @@ void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, 
   S.counter = p->d_counter;
 }
 
-hcx_status plan_launch(const Launch& LN, int mode, int max_n, int max_deg, Plan& pl) {
-  pl.mode = mode;
-  pl.nmax_idx = round_up_idx(kNmaxList, 4, max_n);
-  pl.deg_idx = round_up_idx(kDegList, 4, max_deg);
-  if (pl.nmax_idx < 0 || pl.deg_idx < 0) {
-    set_error("unsupported sizes: n=%d (max %d), degree=%d (max 6)", max_n, kMaxN, max_deg);
-    return HCX_ERR_UNSUPPORTED;
-  }
-  pl.fn = pick_kernel(kNmaxList[pl.nmax_idx], kDegList[pl.deg_idx], mode);
-  if (!pl.fn) {
-    set_error("no kernel instance for n=%d degree=%d mode=%d", max_n, max_deg, mode);
-    return HCX_ERR_UNSUPPORTED;
-  }
+hcx_status finish_plan(const Launch& LN, Plan& pl) {
   size_t lds = 0;
   for (int s = 0; s < LN.nseg; ++s) {
     const Seg& S = LN.seg[s];
@@ -568,12 +608,19 @@ static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batc
     max_n = std::max(max_n, p->n);
     max_deg = std::max(max_deg, p->deg);
   }
-  const int nmax_idx = round_up_idx(kNmaxList, 4, max_n);
-  if (nmax_idx < 0) {
-    set_error("track: n=%d exceeds %d", max_n, kMaxN);
-    return HCX_ERR_UNSUPPORTED;
+  int ns[kMaxSeg], Gs[kMaxSeg];
+  for (int s = 0; s < nseg; ++s) {
+    ns[s] = probs[s]->n;
+    Gs[s] = o->lanes_per_path > 0 ? o->lanes_per_path : default_lanes(probs[s]->n);
+    if (Gs[s] < ns[s]) {
+      set_error("track: lanes_per_path=%d is below n=%d", Gs[s], ns[s]);
+      return HCX_ERR_INVALID_ARG;
+    }
   }
-  const int nmax = kNmaxList[nmax_idx];
+  Plan pl;
+  st = choose_kernel(nseg, ns, Gs, max_deg, o->precision, o->specialize, pl);
+  if (st != HCX_OK) return st;
+  const int nmax = pl.nmax;
   long long total_groups_needed = 0;
   std::vector<long long> seg_groups(nseg);
   for (int s = 0; s < nseg; ++s) {
@@ -584,11 +631,7 @@ static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batc
       set_error("track: segment %d: bad batch or result descriptor", s);
       return HCX_ERR_INVALID_ARG;
     }
-    int G = o->lanes_per_path > 0 ? o->lanes_per_path : default_lanes(p->n);
-    if (G < p->n) {
-      set_error("track: lanes_per_path=%d is below n=%d", G, p->n);
-      return HCX_ERR_INVALID_ARG;
-    }
+    const int G = Gs[s];
     Seg& S = LN.seg[s];
     SegOut& O = OP.out[s];
     seg_from_problem(S, O, p, G, o->precision, nmax);
@@ -637,8 +680,7 @@ static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batc
     seg_groups[s] = (S.P + gpw - 1) / gpw;
     total_groups_needed += seg_groups[s];
   }
-  Plan pl;
-  st = plan_launch(LN, o->precision, max_n, max_deg, pl);
+  st = finish_plan(LN, pl);
   if (st != HCX_OK) return st;
 
   // grid: persistent -> occupancy-limited, split over segments by work
@@ -736,10 +778,11 @@ double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, 
   LN.persistent = 0;
   LN.o = to_opts(&o);
   const int G = default_lanes(p->n);
-  const int nmax_idx = round_up_idx(kNmaxList, 4, p->n);
+  Plan pl;
+  if (choose_kernel(1, &p->n, &G, p->deg, HCX_PREC_FP64, 1, pl) != HCX_OK) return -1.0;
   Seg& S = LN.seg[0];
   SegOut& O = OP.out[0];
-  seg_from_problem(S, O, p, G, HCX_PREC_FP64, kNmaxList[nmax_idx]);
+  seg_from_problem(S, O, p, G, HCX_PREC_FP64, pl.nmax);
   S.start = reinterpret_cast<const double2*>(d_start_sols);
   S.start_stride = p->n + 1;
   O.roots = reinterpret_cast<double2*>(d_track);
@@ -768,8 +811,7 @@ double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, 
   const int gpw = 64 / G;
   LN.first_block[0] = 0;
   LN.first_block[1] = (p->K + gpw - 1) / gpw;
-  Plan pl;
-  if (plan_launch(LN, HCX_PREC_FP64, p->n, p->deg, pl) != HCX_OK) return -1.0;
+  if (finish_plan(LN, pl) != HCX_OK) return -1.0;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     set_error("hcx_kernel_HC_Solver: hipEventCreate failed");

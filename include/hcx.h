@@ -73,6 +73,7 @@ typedef struct {
   int precision;          /* hcx_precision */
   int lanes_per_path;     /* lanes of a wave64 per path (G); 0 = autotuned / n */
   int waves_per_cu;       /* persistent grid size per CU; 0 = from occupancy */
+  int specialize;         /* 1: size-specialised kernel instance when one exists (n = G); 0: generic */
 } hcx_track_opts;
 
 void hcx_track_opts_default(hcx_track_opts* o);
