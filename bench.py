@@ -86,11 +86,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--config", type=int, default=2,
+                    help="workload: 2 (default, the headline line), 1, 3, 4 or 5")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.config != 2:
+        return {1: bench_config1, 3: bench_config3, 4: bench_config4, 5: bench_config5}[args.config](args)
 
     import numpy as np
     import torch
@@ -246,6 +250,273 @@ def main():
                              "counters; peak = vector FP64 (= non-packed FP32 rate)"},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out))
+
+
+def _roofline(flops, kernel_ms):
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_VALU_TFLOPS, "traffic": None, "kernel_ms": kernel_ms, "algorithmic_flop": flops}
+
+
+def _rel_err(a, b):
+    import numpy as np
+    return np.abs(a - b).max(axis=1) / np.maximum(1.0, np.abs(b).max(axis=1))
+
+
+def bench_config1(args):
+    """Dense quadratic n=3 total-degree homotopy: single-instance latency (8
+    paths) of the FP64 per-problem launch and of the mixed launch, and
+    throughput over R instances of the same support with independent seeds."""
+    import numpy as np
+    import torch
+    import hcx
+    import oracle
+    from hcx import workloads
+    c = workloads.config1(args.seed)
+    td = c["homotopy"]
+    hxi, hti = td.tables()
+    nc = td.info["num_coeffs"]
+    prob = hcx.Problem(3, nc, hxi, hti, c["start"])
+    stream = torch.cuda.current_stream()
+    out = {"metric": METRIC, "config": {"workload": "config 1: dense quadratic n=3 total-degree homotopy, 8 paths"}}
+    for name, prec, static in (("fp64_per_problem", hcx.PREC_FP64, True), ("mixed", hcx.PREC_MIXED, False)):
+        opts = hcx.default_opts(precision=prec)
+        b = hcx.Batch(prob, 1, coeffs_hx=c["coeffs_hx"], coeffs_ht=c["coeffs_ht"])
+        r = hcx.Result(8, 3, 1)
+        lat = []
+        for i in range(args.warmup + 50):
+            t0 = time.perf_counter()
+            hcx.track(prob, b, opts, result=r, static=static, stream=stream)
+            torch.cuda.synchronize()
+            if i >= args.warmup:
+                lat.append((time.perf_counter() - t0) * 1e3)
+        h = r.to_host()
+        out[name] = {"p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+                     "refined": int(((h["flags"] & hcx.FLAG_REFINED) != 0).sum())}
+    ref = oracle.OracleProblem(3, nc, hxi, hti, 1).track(c["start"], c["coeffs_hx"], c["coeffs_ht"], threads=1)
+    out["oracle_refined"] = int(((ref["flags"] & oracle.FLAG_REFINED) != 0).sum())
+    out["max_root_rel_err_vs_oracle"] = float(_rel_err(h["roots"], ref["roots"]).max())
+    # throughput: R instances of the dense support, independent coefficients and gamma
+    R = 10000
+    hxs, hts = [], []
+    for i in range(R):
+        sysf = hcx.System.synthetic(3, 0, num_quadratic=3, monomials=0, seed=1000 + i)
+        _, hx, ht = sysf.total_degree(sysf.eval_coeffs(None), hcx.random_gamma(5000 + i))
+        hxs.append(hx)
+        hts.append(ht)
+    b = hcx.Batch(prob, R, coeffs_hx=np.stack(hxs), coeffs_ht=np.stack(hts))
+    r = hcx.Result(R * 8, 3, R)
+    opts = hcx.default_opts(precision=hcx.PREC_MIXED)
+    hcx.track(prob, b, opts, result=r, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        hcx.track(prob, b, opts, result=r, stream=stream)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / args.steps
+    cnt = hcx.collect(r, opts, stream=stream)
+    out.update({"value": cnt["refined"] / el, "unit": "converged paths/s", "ms_per_step": el * 1e3,
+                "instances": R, "counts": cnt})
+    print(json.dumps(out))
+
+
+def bench_config3(args):
+    """Uneven batch: five problem types mixed over 4096 instances, one
+    persistent multi-type launch captured as a HIP graph, autotuned shapes."""
+    import numpy as np
+    import torch
+    import hcx
+    from hcx import workloads
+    I = 4096 if args.instances == 1024 else args.instances
+    log("config3: seeding start roots of five types")
+    types = workloads.quadratic_types(workloads.CONFIG3_TYPES, seed=args.seed)
+    kinds = workloads.mixed_instances(len(types), I, args.seed)
+    opts = hcx.default_opts(precision=hcx.PREC_MIXED)
+    probs, batches, tunes = [], [], []
+    for ti, ty in enumerate(types):
+        cnt = int((kinds == ti).sum())
+        prob = hcx.Problem.from_system(ty["system"], ty["start"], p0=ty["p0"])
+        p1 = workloads.targets(ty["k"], cnt, seed=args.seed * 31 + ti)
+        b = hcx.Batch(prob, cnt, target_params=p1)
+        sample = hcx.Batch(prob, min(cnt, max(8, 4096 // ty["K"])), target_params=p1[:max(8, 4096 // ty["K"])])
+        tunes.append(hcx.autotune(prob, sample, opts))
+        probs.append(prob)
+        batches.append(b)
+    log(f"config3: autotuned {[(t['lanes_per_path'], t['specialize'], t['waves_per_cu']) for t in tunes]}")
+    g = hcx.Graph(probs, batches, opts)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        g.launch(stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(stream)
+    for _ in range(args.steps):
+        g.launch(stream)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / args.steps
+    counts = [hcx.collect(r, opts, stream=stream) for r in g.results]
+    conv = sum(c["refined"] for c in counts)
+    paths = sum(b.I * p.num_tracks for p, b in zip(probs, batches))
+    flops = 0.0
+    lat = []
+    start = g.results[0].to_host()["batch_start"][0]
+    for p, r in zip(probs, g.results):
+        h = r.to_host()
+        flops += float(hcx.flops_per_path(p.info, h["steps"], h["newton"], h["refine"]).sum())
+        lat += list((h["inst_done"][: r.I] - start) / DEVICE_CLOCK_HZ * 1e3)
+    # the same batch as separate per-type persistent launches, for comparison
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p, b in zip(probs, batches):
+        hcx.track(p, b, opts, stream=stream)
+    torch.cuda.synchronize()
+    sep_ms = (time.perf_counter() - t0) * 1e3
+    out = {"metric": METRIC, "value": conv / (ms * 1e-3), "unit": "converged paths/s", "ms_per_step": ms,
+           "steps": args.steps, "warmup": args.warmup, "dtype": "c64+c128 mixed",
+           "config": {"workload": f"config 3: uneven batch of 5 quadratic types (n,K)={workloads.CONFIG3_TYPES}, "
+                                  f"{I} instances mixed uniformly, {paths} paths, one graph-captured launch",
+                      "instances_per_type": [b.I for b in batches]},
+           "paths": paths, "converged": conv, "counts_per_type": counts,
+           "autotune": tunes, "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+           "separate_launches_ms": sep_ms, "roofline": _roofline(flops, ms)}
+    print(json.dumps(out))
+
+
+def bench_config4(args):
+    """Mixed-precision tracking at n=12 (6 quadratic + 6 cubic, 20 monomials,
+    k=36), 512 start roots x 2048 instances; per-root error of the mixed form
+    against the FP64 form on every path and against the oracle on a sample."""
+    import numpy as np
+    import torch
+    import hcx
+    import oracle
+    from hcx import workloads
+    I = 2048 if args.instances == 1024 else args.instances
+    sysf, p0 = workloads.config4_system(args.seed)
+    log("config4: seeding 512 start roots")
+    start = workloads.seed_start_roots(sysf, p0, 512, seed=args.seed)
+    p1 = workloads.targets(36, I, seed=args.seed * 11 + 3)
+    prob = hcx.Problem.from_system(sysf, start, p0=p0)
+    batch = hcx.Batch(prob, I, target_params=p1)
+    stream = torch.cuda.current_stream()
+    out = {"metric": METRIC, "config": {"workload": f"config 4: n=12 (6 quadratic + 6 cubic, 20 monomials, k=36), "
+                                                    f"512 start roots x {I} instances = {512 * I} paths"}}
+    res = {}
+    for name, prec in (("mixed", hcx.PREC_MIXED), ("fp64", hcx.PREC_FP64)):
+        opts = hcx.default_opts(precision=prec)
+        r = hcx.Result(I * 512, 12, I)
+        hcx.track(prob, batch, opts, result=r, stream=stream)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(stream)
+        for _ in range(args.steps):
+            hcx.track(prob, batch, opts, result=r, stream=stream)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / args.steps
+        h = r.to_host()
+        cnt = hcx.collect(r, opts, stream=stream)
+        flops = float(hcx.flops_per_path(prob.info, h["steps"], h["newton"], h["refine"]).sum())
+        res[name] = h
+        out[name] = {"ms_per_launch": ms, "converged_paths_per_s": cnt["refined"] / (ms * 1e-3), "counts": cnt,
+                     "escalated": int(((h["flags"] & hcx.FLAG_ESCALATED) != 0).sum()),
+                     "roofline": _roofline(flops, ms)}
+        log(f"config4: {name} {ms:.2f} ms, counts {cnt}")
+    okm = (res["mixed"]["flags"] & hcx.FLAG_REFINED) != 0
+    ok6 = (res["fp64"]["flags"] & hcx.FLAG_REFINED) != 0
+    both = okm & ok6
+    err = _rel_err(res["mixed"]["roots"][both], res["fp64"]["roots"][both])
+    out["mixed_vs_fp64"] = {"counts_identical": bool(okm.sum() == ok6.sum()), "refined": [int(okm.sum()), int(ok6.sum())],
+                            "max_root_rel_err": float(err.max()), "median_root_rel_err": float(np.median(err))}
+    # oracle on a bounded sample of instances
+    ns = 4
+    hxi, hti = sysf.tables()
+    op = oracle.OracleProblem(12, sysf.info["num_coeffs"], hxi, hti, 1)
+    ohx, oht = oracle.p2c_affine(sysf.affine(), p0, p1[:ns])
+    t0 = time.perf_counter()
+    o = op.track(start, ohx, oht, threads=os.cpu_count() or 1)
+    el = time.perf_counter() - t0
+    okr = (o["flags"] & oracle.FLAG_REFINED) != 0
+    sm = okm[: ns * 512]
+    b2 = sm & okr
+    e2 = _rel_err(res["mixed"]["roots"][: ns * 512][b2], o["roots"][b2])
+    out["mixed_vs_oracle_sample"] = {"instances": ns, "paths": ns * 512, "refined_gpu": int(sm.sum()),
+                                     "refined_oracle": int(okr.sum()), "max_root_rel_err": float(e2.max()),
+                                     "oracle_seconds": el, "oracle_threads": os.cpu_count()}
+    out["value"] = out["mixed"]["converged_paths_per_s"]
+    out["unit"] = "converged paths/s"
+    out["speedup_mixed_vs_fp64_batched"] = out["fp64"]["ms_per_launch"] / out["mixed"]["ms_per_launch"]
+    print(json.dumps(out))
+
+
+def bench_config5(args):
+    """Serving loop at a fixed Poisson request rate (2000 req/s by default):
+    six resident quadratic types n in {3,6,9,12,16,30}, one parameter instance
+    per request, continuous batching with admission by HBM budget."""
+    import numpy as np
+    import hcx
+    from hcx import workloads
+    rate = float(os.environ.get("HCX_SERVE_RATE", "2000"))
+    duration = float(os.environ.get("HCX_SERVE_SECONDS", "30"))
+    log("config5: seeding start roots of six types")
+    types = workloads.quadratic_types(workloads.CONFIG5_TYPES, seed=args.seed)
+    probs = [hcx.Problem.from_system(t["system"], t["start"], p0=t["p0"]) for t in types]
+    opts = hcx.default_opts(precision=hcx.PREC_MIXED)
+    srv = hcx.Server(opts, hbm_budget_bytes=4 << 30, max_batch_requests=64, streams=4)
+    tids = [srv.add_type(p) for p in probs]
+    rng = np.random.default_rng(args.seed)
+    # warm-up: every type once
+    for ti, ty in enumerate(types):
+        srv.submit(tids[ti], workloads.targets(ty["k"], 1, 90000 + ti)[0])
+    srv.drain()
+    srv.reset_stats()
+    n_req = int(rate * duration)
+    gaps = rng.exponential(1.0 / rate, n_req)
+    kinds = rng.integers(0, len(types), n_req)
+    params = [workloads.targets(types[k]["k"], 1, 100000 + i)[0] for i, k in enumerate(kinds)]
+    pending = []
+    t_start = time.perf_counter()
+    t_next = t_start
+    late = 0.0
+    log(f"config5: {n_req} requests at {rate:.0f}/s for {duration:.0f} s")
+    last_log = t_start
+    for i in range(n_req):
+        t_next += gaps[i]
+        while True:
+            now = time.perf_counter()
+            if now >= t_next:
+                break
+            if pending and t_next - now > 0.0005:
+                rid, k = pending[0]
+                if srv.result(rid, k) is not None:
+                    pending.pop(0)
+                    continue
+            if t_next - now > 0.0003:
+                time.sleep(0.0001)
+        late = max(late, time.perf_counter() - t_next)
+        pending.append((srv.submit(tids[kinds[i]], params[i]), int(kinds[i])))
+        if time.perf_counter() - last_log > 5:
+            last_log = time.perf_counter()
+            st = srv.stats()
+            log(f"config5: t={last_log - t_start:.1f}s submitted {st['submitted']} completed {st['completed']} "
+                f"inflight {st['inflight_requests']} p99 {st['p99_ms']:.2f} ms")
+    t_sub = time.perf_counter() - t_start
+    srv.drain(300.0)
+    t_end = time.perf_counter() - t_start
+    for rid, k in pending:
+        srv.result(rid, k)
+    st = srv.stats()
+    out = {"metric": METRIC, "value": st["converged_paths"] / t_end, "unit": "converged paths/s",
+           "config": {"workload": f"config 5: serving loop, Poisson arrivals at {rate:.0f} req/s for {duration:.0f} s, "
+                                  f"types (n,K)={workloads.CONFIG5_TYPES} drawn uniformly, mixed FP32/FP64",
+                      "hbm_budget_bytes": 4 << 30, "streams": 4, "max_batch_requests": 64},
+           "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "max_ms": st["max_ms"],
+           "requests_per_s": st["completed"] / t_end, "offered_requests_per_s": rate,
+           "submit_seconds": t_sub, "total_seconds": t_end, "max_submit_lateness_ms": late * 1e3,
+           "stats": st}
+    srv.close()
     print(json.dumps(out))
 
 

@@ -6,7 +6,7 @@
 #include <type_traits>
 
 #include "hcx_device.hpp"
-#include "hcx.h"
+#include "hcx_flags.h"
 
 namespace hcx {
 
@@ -33,6 +33,14 @@ struct SegOut {
   unsigned long long* inst_done;
   int* inst_left;
   int roots_stride;
+};
+
+// Per-launch device workspace of one segment (path queue, output
+// descriptor, per-instance remaining-path counters).
+struct LaunchWork {
+  unsigned int* counter;
+  SegOut* out;
+  int* inst_left;
 };
 
 // Hot, wave-uniform description of one segment (kernel arguments).

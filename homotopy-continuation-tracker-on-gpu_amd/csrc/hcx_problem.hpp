@@ -1,0 +1,36 @@
+// Problem object and the shared launch entry of libhcx (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hcx.h"
+#include "hcx_kernel.hpp"
+
+// Device-resident problem type: repacked tables, start roots, affine map and
+// the default launch workspace.
+struct hcx_problem {
+  int device = 0;
+  int n = 0, C = 0, M = 1, K = 0, k = 0;
+  int hxT = 0, htT = 0, hx_vars = 0, ht_vars = 0, deg = 2;
+  hcx::dev::Term* d_hx = nullptr;
+  hcx::dev::Term* d_ht = nullptr;
+  int* d_hx_cnt = nullptr;
+  double2* d_start = nullptr;
+  double2* d_affine = nullptr;
+  double2* d_p0 = nullptr;
+  // launch workspace
+  unsigned int* d_counter = nullptr;
+  hcx::SegOut* d_out = nullptr;
+  int* d_inst_left = nullptr;
+  int inst_cap = 0;
+  // autotuned launch shape (hcx_autotune); used when the options leave it open
+  bool tuned = false;
+  int tuned_G = 0, tuned_spec = 1, tuned_wpc = 0;
+  hcx_problem_info info{};
+};
+
+namespace hcx {
+hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc* batches,
+                      const hcx_result_desc* results, const hcx_track_opts* o, hipStream_t stream, int persistent,
+                      unsigned long long* batch_start, const LaunchWork* work);
+}  // namespace hcx

@@ -162,24 +162,8 @@ static KernelFn pick_exact(int n, int deg, int mode) {
 
 }  // namespace hcx
 
-// ============================================================== problem object
-struct hcx_problem {
-  int device = 0;
-  int n = 0, C = 0, M = 1, K = 0, k = 0;
-  int hxT = 0, htT = 0, hx_vars = 0, ht_vars = 0, deg = 2;
-  hcx::dev::Term* d_hx = nullptr;
-  hcx::dev::Term* d_ht = nullptr;
-  int* d_hx_cnt = nullptr;
-  double2* d_start = nullptr;
-  double2* d_affine = nullptr;
-  double2* d_p0 = nullptr;
-  // launch workspace
-  unsigned int* d_counter = nullptr;
-  hcx::SegOut* d_out = nullptr;
-  int* d_inst_left = nullptr;
-  int inst_cap = 0;
-  hcx_problem_info info{};
-};
+#include "hcx_problem.hpp"
+
 
 namespace hcx {
 namespace {
@@ -575,9 +559,15 @@ hcx_status hcx_sync(void* stream) {
 }
 
 // Shared implementation of hcx_track / hcx_track_multi.
-static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc* batches,
-                             const hcx_result_desc* results, const hcx_track_opts* o, hipStream_t stream,
-                             int persistent, unsigned long long* batch_start) {
+}  // extern "C"
+
+namespace hcx {
+// Shared implementation of hcx_track / hcx_track_multi / graphs / the server.
+// `work` (optional, one per segment) replaces the problem-owned launch
+// workspace so that several launches of one problem can be in flight.
+hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc* batches,
+                      const hcx_result_desc* results, const hcx_track_opts* o, hipStream_t stream, int persistent,
+                      unsigned long long* batch_start, const LaunchWork* work) {
   hcx_status st = validate_opts(o);
   if (st != HCX_OK) return st;
   if (nseg < 1 || nseg > kMaxSeg || !probs || !batches || !results) {
@@ -609,16 +599,22 @@ static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batc
     max_deg = std::max(max_deg, p->deg);
   }
   int ns[kMaxSeg], Gs[kMaxSeg];
+  int specialize = o->specialize;
+  bool all_tuned = o->lanes_per_path == 0;
   for (int s = 0; s < nseg; ++s) {
+    all_tuned = all_tuned && probs[s]->tuned;
     ns[s] = probs[s]->n;
-    Gs[s] = o->lanes_per_path > 0 ? o->lanes_per_path : default_lanes(probs[s]->n);
+    if (o->lanes_per_path > 0) Gs[s] = o->lanes_per_path;
+    else if (probs[s]->tuned) Gs[s] = probs[s]->tuned_G;
+    else Gs[s] = default_lanes(probs[s]->n);
     if (Gs[s] < ns[s]) {
       set_error("track: lanes_per_path=%d is below n=%d", Gs[s], ns[s]);
       return HCX_ERR_INVALID_ARG;
     }
   }
+  if (all_tuned && nseg == 1) specialize = probs[0]->tuned_spec;
   Plan pl;
-  st = choose_kernel(nseg, ns, Gs, max_deg, o->precision, o->specialize, pl);
+  st = choose_kernel(nseg, ns, Gs, max_deg, o->precision, specialize, pl);
   if (st != HCX_OK) return st;
   const int nmax = pl.nmax;
   long long total_groups_needed = 0;
@@ -671,7 +667,11 @@ static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batc
     O.dtfin = r.d_dt_final;
     O.resid = r.d_residual;
     O.inst_done = reinterpret_cast<unsigned long long*>(r.d_inst_done);
-    if (r.d_inst_done) {
+    if (work) {
+      S.counter = work[s].counter;
+      S.out = work[s].out;
+      O.inst_left = r.d_inst_done ? work[s].inst_left : nullptr;
+    } else if (r.d_inst_done) {
       st = ensure_inst_cap(p, b.num_instances);
       if (st != HCX_OK) return st;
       O.inst_left = p->d_inst_left;
@@ -687,6 +687,7 @@ static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batc
   long long blocks;
   if (persistent) {
     int per_cu = o->waves_per_cu;
+    if (per_cu <= 0 && all_tuned && nseg == 1) per_cu = probs[0]->tuned_wpc;
     if (per_cu <= 0) {
       int nb = 0;
       HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(pl.fn), 64, pl.lds));
@@ -729,6 +730,10 @@ static hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batc
   return HCX_OK;
 }
 
+}  // namespace hcx
+
+extern "C" {
+
 hcx_status hcx_track(hcx_problem* p, const hcx_batch_desc* b, const hcx_track_opts* o, const hcx_result_desc* r,
                      void* stream) {
   clear_error();
@@ -736,7 +741,7 @@ hcx_status hcx_track(hcx_problem* p, const hcx_batch_desc* b, const hcx_track_op
     set_error("hcx_track: null argument");
     return HCX_ERR_INVALID_ARG;
   }
-  return track_impl(1, &p, b, r, o, (hipStream_t)stream, 1, reinterpret_cast<unsigned long long*>(r->d_batch_start));
+  return track_impl(1, &p, b, r, o, (hipStream_t)stream, 1, reinterpret_cast<unsigned long long*>(r->d_batch_start), nullptr);
 }
 
 hcx_status hcx_track_multi(int num_segments, hcx_problem* const* problems, const hcx_batch_desc* batches,
@@ -747,7 +752,7 @@ hcx_status hcx_track_multi(int num_segments, hcx_problem* const* problems, const
     return HCX_ERR_INVALID_ARG;
   }
   return track_impl(num_segments, problems, batches, results, o, (hipStream_t)stream, 1,
-                    reinterpret_cast<unsigned long long*>(results[0].d_batch_start));
+                    reinterpret_cast<unsigned long long*>(results[0].d_batch_start), nullptr);
 }
 
 double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, int max_corr, int incr_steps,
@@ -843,7 +848,7 @@ hcx_status hcx_track_static(hcx_problem* p, const hcx_batch_desc* b, const hcx_t
     set_error("hcx_track_static: null argument");
     return HCX_ERR_INVALID_ARG;
   }
-  return track_impl(1, &p, b, r, o, (hipStream_t)stream, 0, reinterpret_cast<unsigned long long*>(r->d_batch_start));
+  return track_impl(1, &p, b, r, o, (hipStream_t)stream, 0, reinterpret_cast<unsigned long long*>(r->d_batch_start), nullptr);
 }
 
 hcx_status hcx_collect(int n, int num_instances, int num_tracks, const hcx_complex* d_roots, uint32_t* d_flags,
@@ -870,6 +875,210 @@ hcx_status hcx_collect(int n, int num_instances, int num_tracks, const hcx_compl
   out->refined = (int64_t)h[2];
   out->real = (int64_t)h[3];
   out->unique = (int64_t)h[4];
+  return HCX_OK;
+}
+
+}  // extern "C"
+
+// ============================================================== autotuner + graphs
+extern "C" {
+
+// Times candidate launch shapes of problem p on a sample batch and keeps the
+// fastest: lanes per path G in {n, next power of two <= 64}, the
+// size-specialised or the generic kernel instance, and the persistent grid
+// density (occupancy, half occupancy).  Results are bitwise-independent of
+// the grid density; G and the kernel variant change only the summation order
+// of group reductions.
+hcx_status hcx_autotune(hcx_problem* p, const hcx_batch_desc* sample, const hcx_track_opts* o,
+                        hcx_tune_result* out, void* stream) {
+  clear_error();
+  if (!p || !sample || !o || sample->num_instances < 1) {
+    set_error("hcx_autotune: need a problem, options and a non-empty sample batch");
+    return HCX_ERR_INVALID_ARG;
+  }
+  hcx_status st = validate_opts(o);
+  if (st != HCX_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const long long P = (long long)sample->num_instances * p->K;
+  double2* roots = nullptr;
+  uint32_t* flags = nullptr;
+  HIP_TRY(hipMalloc(&roots, sizeof(double2) * P * p->n));
+  HIP_TRY(hipMalloc(&flags, sizeof(uint32_t) * P));
+  hcx_result_desc r;
+  std::memset(&r, 0, sizeof(r));
+  r.d_roots = reinterpret_cast<hcx_complex*>(roots);
+  r.d_flags = flags;
+  int pow2 = 1;
+  while (pow2 < p->n) pow2 <<= 1;
+  struct Cand { int G, spec, wpc; };
+  std::vector<Cand> cands;
+  int occ = 0;
+  {
+    // occupancy of the default shape, to derive the half-density candidate
+    Plan pl;
+    int G = p->n;
+    if (choose_kernel(1, &p->n, &G, p->deg, o->precision, 1, pl) == HCX_OK) {
+      Launch LN;
+      std::memset(&LN, 0, sizeof(LN));
+      SegOut O;
+      LN.nseg = 1;
+      seg_from_problem(LN.seg[0], O, p, G, o->precision, pl.nmax);
+      if (finish_plan(LN, pl) == HCX_OK)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(pl.fn), 64, pl.lds);
+    }
+  }
+  for (int spec = 1; spec >= 0; --spec) {
+    cands.push_back({p->n, spec, 0});
+    if (occ > 1) cands.push_back({p->n, spec, std::max(1, occ / 2)});
+  }
+  if (pow2 != p->n && pow2 <= 64) {
+    cands.push_back({pow2, 0, 0});
+    if (occ > 1) cands.push_back({pow2, 0, std::max(1, occ / 2)});
+  }
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  double best = 1e300;
+  Cand bc = cands[0];
+  const bool was_tuned = p->tuned;
+  p->tuned = false;
+  int timed = 0;
+  for (const Cand& c : cands) {
+    hcx_track_opts oc = *o;
+    oc.lanes_per_path = c.G;
+    oc.specialize = c.spec;
+    oc.waves_per_cu = c.wpc;
+    st = track_impl(1, &p, sample, &r, &oc, s, 1, nullptr, nullptr);  // warm-up
+    if (st != HCX_OK) continue;
+    HIP_TRY(hipEventRecord(e0, s));
+    const int reps = 2;
+    for (int q = 0; q < reps && st == HCX_OK; ++q) st = track_impl(1, &p, sample, &r, &oc, s, 1, nullptr, nullptr);
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
+    if (st != HCX_OK) continue;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    ++timed;
+    if (ms / reps < best) {
+      best = ms / reps;
+      bc = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(roots);
+  (void)hipFree(flags);
+  if (timed == 0) {
+    p->tuned = was_tuned;
+    char msg[512];
+    std::snprintf(msg, sizeof(msg), "%s", hcx_last_error());
+    set_error("hcx_autotune: no candidate ran (%s)", msg);
+    return HCX_ERR_UNSUPPORTED;
+  }
+  p->tuned = true;
+  p->tuned_G = bc.G;
+  p->tuned_spec = bc.spec;
+  p->tuned_wpc = bc.wpc;
+  if (out) {
+    out->lanes_per_path = bc.G;
+    out->specialize = bc.spec;
+    out->waves_per_cu = bc.wpc;
+    out->best_ms = best;
+    out->candidates = timed;
+  }
+  return HCX_OK;
+}
+
+hcx_status hcx_problem_set_launch_shape(hcx_problem* p, int lanes_per_path, int specialize, int waves_per_cu) {
+  clear_error();
+  if (!p || lanes_per_path < 0 || lanes_per_path > 64 || (lanes_per_path > 0 && lanes_per_path < p->n) ||
+      specialize < 0 || specialize > 1 || waves_per_cu < 0) {
+    set_error("hcx_problem_set_launch_shape: invalid shape");
+    return HCX_ERR_INVALID_ARG;
+  }
+  p->tuned = lanes_per_path > 0;
+  p->tuned_G = lanes_per_path;
+  p->tuned_spec = specialize;
+  p->tuned_wpc = waves_per_cu;
+  return HCX_OK;
+}
+
+}  // extern "C"
+
+struct hcx_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+// Captures one multi-segment persistent launch (counter set-up, device P2C,
+// tracking) into a HIP graph.  Counters are cleared by a kernel, not a memset
+// node.  The arguments are baked into the graph: the same device buffers are
+// reused on every hcx_graph_launch.
+hcx_status hcx_graph_capture_multi(int num_segments, hcx_problem* const* problems, const hcx_batch_desc* batches,
+                                   const hcx_result_desc* results, const hcx_track_opts* o, hcx_graph** out) {
+  clear_error();
+  if (!out || num_segments < 1 || !results) {
+    set_error("hcx_graph_capture_multi: invalid arguments");
+    return HCX_ERR_INVALID_ARG;
+  }
+  // an uncaptured run first: allocates per-problem workspace and sets kernel
+  // attributes outside the capture
+  hipStream_t cs = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hcx_status st = track_impl(num_segments, problems, batches, results, o, cs, 1,
+                             reinterpret_cast<unsigned long long*>(results[0].d_batch_start), nullptr);
+  if (st != HCX_OK) {
+    (void)hipStreamDestroy(cs);
+    return st;
+  }
+  HIP_TRY(hipStreamSynchronize(cs));
+  auto* g = new hcx_graph();
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    st = track_impl(num_segments, problems, batches, results, o, cs, 1,
+                    reinterpret_cast<unsigned long long*>(results[0].d_batch_start), nullptr);
+    hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
+    if (st == HCX_OK && e2 != hipSuccess) {
+      set_error("hipStreamEndCapture: %s", hipGetErrorString(e2));
+      st = HCX_ERR_HIP;
+    }
+  } else {
+    set_error("hipStreamBeginCapture: %s", hipGetErrorString(e));
+    st = HCX_ERR_HIP;
+  }
+  if (st == HCX_OK) {
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+      st = HCX_ERR_HIP;
+    }
+  }
+  (void)hipStreamDestroy(cs);
+  if (st != HCX_OK) {
+    hcx_graph_destroy(g);
+    return st;
+  }
+  *out = g;
+  return HCX_OK;
+}
+
+hcx_status hcx_graph_launch(hcx_graph* g, void* stream) {
+  clear_error();
+  if (!g || !g->exec) {
+    set_error("hcx_graph_launch: no graph");
+    return HCX_ERR_INVALID_ARG;
+  }
+  HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return HCX_OK;
+}
+
+hcx_status hcx_graph_destroy(hcx_graph* g) {
+  if (!g) return HCX_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
   return HCX_OK;
 }
 

@@ -343,6 +343,105 @@ def track_multi(problems, batches, opts, results=None, stream=None):
     return results
 
 
+def autotune(problem, batch, opts, stream=None):
+    """Times launch shapes on `batch` and stores the fastest in the problem."""
+    r = _capi.TuneResult()
+    b = batch.desc()
+    _check(lib().hcx_autotune(problem.handle, C.byref(b), C.byref(opts), C.byref(r), _stream_handle(stream)),
+           "hcx_autotune")
+    return {f: getattr(r, f) for f, _ in _capi.TuneResult._fields_}
+
+
+class Graph:
+    """One multi-type batch launch captured as a HIP graph; replays reuse the
+    batch and result buffers given at capture."""
+
+    def __init__(self, problems, batches, opts, results=None):
+        if results is None:
+            results = [Result(b.I * p.num_tracks, p.n, b.I) for p, b in zip(problems, batches)]
+        self.problems, self.batches, self.results, self.opts = problems, batches, results, opts
+        ns = len(problems)
+        self._ph = (C.c_void_p * ns)(*[p.handle for p in problems])
+        self._bd = (BatchDesc * ns)(*[b.desc() for b in batches])
+        self._rd = (ResultDesc * ns)(*[r.desc() for r in results])
+        h = C.c_void_p()
+        _check(lib().hcx_graph_capture_multi(ns, self._ph, self._bd, self._rd, C.byref(opts), C.byref(h)),
+               "hcx_graph_capture_multi")
+        self._h = h
+
+    def launch(self, stream=None):
+        _check(lib().hcx_graph_launch(self._h, _stream_handle(stream)), "hcx_graph_launch")
+        return self.results
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.hcx_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Server:
+    """Continuous-batching solve server over resident problem types (libhcx)."""
+
+    def __init__(self, track_opts, hbm_budget_bytes=1 << 30, max_batch_requests=64, streams=4,
+                 max_queue=1 << 20):
+        so = _capi.ServerOpts()
+        lib().hcx_server_opts_default(C.byref(so))
+        so.hbm_budget_bytes, so.max_batch_requests, so.streams, so.max_queue = (
+            hbm_budget_bytes, max_batch_requests, streams, max_queue)
+        self._opts = track_opts
+        h = C.c_void_p()
+        _check(lib().hcx_server_create(C.byref(so), C.byref(track_opts), C.byref(h)), "hcx_server_create")
+        self._h = h
+        self.types = []
+
+    def add_type(self, problem):
+        tid = C.c_int()
+        _check(lib().hcx_server_add_type(self._h, problem.handle, C.byref(tid)), "hcx_server_add_type")
+        self.types.append(problem)
+        return tid.value
+
+    def submit(self, type_id, params):
+        p = np.ascontiguousarray(params, dtype=np.complex128)
+        rid = C.c_uint64()
+        _check(lib().hcx_server_submit(self._h, type_id, _ptr(p), C.byref(rid)), "hcx_server_submit")
+        return rid.value
+
+    def result(self, request_id, type_id):
+        """Blocking-free fetch: returns None until done, else (info dict, roots, flags)."""
+        prob = self.types[type_id]
+        roots = np.empty((prob.num_tracks, prob.n), dtype=np.complex128)
+        flags = np.empty(prob.num_tracks, dtype=np.uint32)
+        done = C.c_int()
+        rr = _capi.RequestResult()
+        _check(lib().hcx_server_poll(self._h, C.c_uint64(request_id), C.byref(done), C.byref(rr), _ptr(roots),
+                                     _ptr(flags)), "hcx_server_poll")
+        if not done.value:
+            return None
+        return {f: getattr(rr, f) for f, _ in _capi.RequestResult._fields_}, roots, flags
+
+    def drain(self, timeout_s=120.0):
+        _check(lib().hcx_server_drain(self._h, timeout_s), "hcx_server_drain")
+
+    def stats(self):
+        st = _capi.ServerStats()
+        _check(lib().hcx_server_get_stats(self._h, C.byref(st)), "hcx_server_get_stats")
+        return {f: getattr(st, f) for f, _ in _capi.ServerStats._fields_}
+
+    def reset_stats(self):
+        lib().hcx_server_reset_stats(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.hcx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 def collect(result, opts, stream=None):
     c = Counts()
     _check(lib().hcx_collect(result.n, result.I, result.P // max(result.I, 1), C.c_void_p(result.roots.data_ptr()),

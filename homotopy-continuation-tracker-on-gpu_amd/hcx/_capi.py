@@ -81,9 +81,14 @@ class RefSettings(C.Structure):
                 ("max_order_t", C.c_int), ("num_coeffs", C.c_int)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("lanes_per_path", C.c_int), ("specialize", C.c_int), ("waves_per_cu", C.c_int),
+                ("best_ms", C.c_double), ("candidates", C.c_int)]
+
+
 class ServerOpts(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_inflight_paths", C.c_int),
-                ("steps_per_launch", C.c_int), ("max_queue", C.c_int)]
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_batch_requests", C.c_int), ("streams", C.c_int),
+                ("max_queue", C.c_int)]
 
 
 class ServerStats(C.Structure):
@@ -96,7 +101,7 @@ class ServerStats(C.Structure):
 
 class RequestResult(C.Structure):
     _fields_ = [("type_id", C.c_int), ("num_paths", C.c_int), ("refined", C.c_int), ("real", C.c_int),
-                ("unique", C.c_int), ("latency_ms", C.c_double), ("queue_ms", C.c_double)]
+                ("latency_ms", C.c_double), ("queue_ms", C.c_double)]
 
 
 # name -> (restype, argtypes).  Every symbol here is declared in include/hcx.h.
@@ -118,6 +123,23 @@ SIGNATURES = {
                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p]),
     "hcx_sync": (C.c_int, [C.c_void_p]),
+    "hcx_autotune": (C.c_int, [C.c_void_p, C.POINTER(BatchDesc), C.POINTER(TrackOpts), C.POINTER(TuneResult),
+                               C.c_void_p]),
+    "hcx_problem_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "hcx_graph_capture_multi": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(BatchDesc),
+                                          C.POINTER(ResultDesc), C.POINTER(TrackOpts), C.POINTER(C.c_void_p)]),
+    "hcx_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hcx_graph_destroy": (C.c_int, [C.c_void_p]),
+    "hcx_server_opts_default": (None, [C.POINTER(ServerOpts)]),
+    "hcx_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(TrackOpts), C.POINTER(C.c_void_p)]),
+    "hcx_server_add_type": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
+    "hcx_server_submit": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "hcx_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), C.POINTER(RequestResult), C.c_void_p,
+                                  C.c_void_p]),
+    "hcx_server_drain": (C.c_int, [C.c_void_p, C.c_double]),
+    "hcx_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "hcx_server_reset_stats": (C.c_int, [C.c_void_p]),
+    "hcx_server_destroy": (C.c_int, [C.c_void_p]),
     "hcx_collect": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.POINTER(Counts),
                               C.c_void_p]),
     "hcx_collect_host": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_double,

@@ -69,9 +69,6 @@ def seed_start_roots(system, p0, K, seed, opts=None, margin=1.3, max_rounds=4):
         opts = default_opts(precision=PREC_FP64, max_steps=400)
     coeffs = system.eval_coeffs(p0)
     gamma = random_gamma(_seed(seed, 11))
-    bezout = 1
-    for i in range(system.info["num_vars"]):
-        bezout *= 2 if True else 1
     count = K
     found = np.zeros((0, system.info["num_vars"]), dtype=np.complex128)
     for rnd in range(max_rounds):
@@ -121,3 +118,27 @@ def config4_system(seed=4):
     sysf = System.synthetic(12, 36, num_quadratic=6, monomials=20, seed=_seed(seed, 1))
     p0 = random_cn(_seed(seed, 3), 36)
     return sysf, p0
+
+
+CONFIG3_TYPES = [(3, 8), (5, 32), (6, 64), (9, 142), (12, 583)]
+CONFIG5_TYPES = [(3, 8), (6, 64), (9, 142), (12, 583), (16, 40), (30, 312)]
+
+
+def quadratic_types(types, seed, num_params=None):
+    """Problem types of the uneven batch / serving workloads: per type a sparse
+    quadratic parametric system (k = 2n parameters unless given), its seed
+    parameters p0 and K start roots from the seed total-degree solve."""
+    out = []
+    for ti, (n, K) in enumerate(types):
+        k = num_params if num_params else 2 * n
+        sysf = quadratic_type(n, k, _seed(seed, 100 + ti))
+        p0 = random_cn(_seed(seed, 200 + ti), k)
+        start = seed_start_roots(sysf, p0, K, seed=_seed(seed, 300 + ti))
+        out.append(dict(n=n, K=K, k=k, system=sysf, p0=p0, start=start))
+    return out
+
+
+def mixed_instances(num_types, count, seed):
+    """Type of each instance, uniform at random (SplitMix64 stream)."""
+    u = random_cn(_seed(seed, 400), count)
+    return (np.floor((np.angle(u) + np.pi) / (2 * np.pi) * num_types).astype(np.int64) % num_types)

@@ -145,16 +145,7 @@ typedef struct {
   hcx_complex* d_coeff_work;
 } hcx_batch_desc;
 
-/* Path flag bits. */
-#define HCX_FLAG_TRACKED    0x01u  /* reached t = 1 (reference "converged", kernel ...4pts.cu:258) */
-#define HCX_FLAG_INFINITY   0x02u  /* |x|^2 > inf_tol (reference "infinity", :259) */
-#define HCX_FLAG_REFINED    0x04u  /* FP64 refinement at t = 1 converged to a finite root */
-#define HCX_FLAG_REAL       0x08u  /* refined root has all |Im x_i| <= real_tol */
-#define HCX_FLAG_UNIQUE     0x10u  /* first occurrence of this root in its instance (hcx_collect) */
-#define HCX_FLAG_BUDGET     0x20u  /* max_steps exhausted before t = 1 */
-#define HCX_FLAG_MINSTEP    0x40u  /* dt fell below dt_min */
-#define HCX_FLAG_SINGULAR   0x80u  /* zero pivot met in the final refinement */
-#define HCX_FLAG_ESCALATED  0x100u /* mixed-precision attempt failed; path re-tracked in FP64 */
+#include "hcx_flags.h"
 
 typedef struct {
   hcx_complex* d_roots;      /* [P][n] final x (refined when refine_iters > 0) */
@@ -199,6 +190,86 @@ double hcx_kernel_HC_Solver(void* stream, const hcx_problem* p, int max_steps, i
                             uint8_t* d_converge, uint8_t* d_infinity, hcx_complex* d_debug);
 
 hcx_status hcx_sync(void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Launch-shape autotuner (no counterpart in the reference, whose launch      */
+/* shape is fixed per problem: kernel_HC_Solver_<p>.cu:299-300).  Times       */
+/* lanes-per-path (paths per wavefront = 64 / G), kernel variant and grid     */
+/* density on a sample batch and stores the winner in the problem; later     */
+/* launches use it when hcx_track_opts.lanes_per_path == 0.                  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int lanes_per_path;     /* G: paths per wavefront = floor(64 / G) */
+  int specialize;         /* size-specialised kernel instance (1) or generic (0) */
+  int waves_per_cu;       /* persistent grid density (0 = occupancy) */
+  double best_ms;         /* sample launch time of the chosen shape */
+  int candidates;         /* shapes timed */
+} hcx_tune_result;
+
+hcx_status hcx_autotune(hcx_problem* p, const hcx_batch_desc* sample, const hcx_track_opts* o,
+                        hcx_tune_result* out, void* stream);
+/* Sets the stored launch shape directly (lanes_per_path = 0 clears it). */
+hcx_status hcx_problem_set_launch_shape(hcx_problem* p, int lanes_per_path, int specialize, int waves_per_cu);
+
+/* ------------------------------------------------------------------------ */
+/* HIP graphs: one uneven multi-type batch (hcx_track_multi) captured once   */
+/* and replayed with hcx_graph_launch (same device buffers every replay).    */
+/* ------------------------------------------------------------------------ */
+typedef struct hcx_graph hcx_graph;
+hcx_status hcx_graph_capture_multi(int num_segments, hcx_problem* const* problems, const hcx_batch_desc* batches,
+                                   const hcx_result_desc* results, const hcx_track_opts* o, hcx_graph** out);
+hcx_status hcx_graph_launch(hcx_graph* g, void* stream);
+hcx_status hcx_graph_destroy(hcx_graph* g);
+
+/* ------------------------------------------------------------------------ */
+/* Serving loop: continuous batching of solve requests (one parameter        */
+/* instance of a registered problem type each).  Replaces the reference's    */
+/* one-instance-per-synchronous-launch flow (GPU_HC_Solver::Solve_by_GPU_HC,  */
+/* GPU_HC_Solver.cpp:207-384, driven per RANSAC target by                    */
+/* cmd/magmaHC-main.cpp:93-112).  Requests are queued per type and launched  */
+/* as micro-batches on up to four streams while an HBM budget admits them;   */
+/* tables and start roots stay resident; results are returned per request.   */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int64_t hbm_budget_bytes;   /* device memory for in-flight requests (params, P2C tables, results) */
+  int max_batch_requests;     /* requests per micro-batch launch */
+  int streams;                /* concurrent launches, 1..4 */
+  int max_queue;              /* queued requests before submit rejects */
+} hcx_server_opts;
+
+typedef struct {
+  int64_t submitted, completed, rejected;
+  int64_t inflight_requests, inflight_paths;
+  int64_t launches, paths_done, converged_paths;
+  int64_t hbm_in_use, peak_inflight_paths;
+  double p50_ms, p99_ms, mean_ms, max_ms;   /* submit -> result latency */
+} hcx_server_stats;
+
+typedef struct {
+  int type_id, num_paths, refined, real;
+  double latency_ms;   /* submit -> result available */
+  double queue_ms;     /* submit -> launch */
+} hcx_request_result;
+
+typedef struct hcx_server hcx_server;
+
+void hcx_server_opts_default(hcx_server_opts* o);
+hcx_status hcx_server_create(const hcx_server_opts* so, const hcx_track_opts* to, hcx_server** out);
+/* Registers a resident problem type (needs its parameter-affine map); all
+ * types are added before the first submit. */
+hcx_status hcx_server_add_type(hcx_server* s, hcx_problem* p, int* type_id);
+/* Queues one request: h_target_params holds the type's k parameters.
+ * Thread-safe; returns immediately. */
+hcx_status hcx_server_submit(hcx_server* s, int type_id, const hcx_complex* h_target_params, uint64_t* request_id);
+/* *done = 1 once the request finished; then fills *out and (if non-NULL)
+ * h_roots [K][n] and h_flags [K], and forgets the request. */
+hcx_status hcx_server_poll(hcx_server* s, uint64_t request_id, int* done, hcx_request_result* out,
+                           hcx_complex* h_roots, uint32_t* h_flags);
+/* Waits until every submitted request has finished (or the timeout). */
+hcx_status hcx_server_drain(hcx_server* s, double timeout_s);
+hcx_status hcx_server_get_stats(hcx_server* s, hcx_server_stats* out);
+hcx_status hcx_server_reset_stats(hcx_server* s);
+hcx_status hcx_server_destroy(hcx_server* s);
 
 /* ------------------------------------------------------------------------ */
 /* Solution collection (GPU-HC/magmaHC/Evaluations.cpp:66-176).  Sets        */
