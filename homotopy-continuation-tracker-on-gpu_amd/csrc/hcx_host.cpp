@@ -160,13 +160,14 @@ void hcx_track_opts_default(hcx_track_opts* o) {
   o->corr_tol = 1e-6;
   o->inf_tol = 1e14;
   o->refine_iters = 8;
-  o->refine_tol = 1e-13;
+  o->refine_tol = 1e-10;
   o->real_tol = 1e-8;
   o->unique_tol = 1e-4;
   o->precision = HCX_PREC_FP64;
   o->lanes_per_path = 0;
   o->waves_per_cu = 0;
   o->specialize = 1;
+  o->retrack_duplicates = 1;
 }
 
 hcx_status hcx_random_cn(uint64_t seed, int64_t count, hcx_complex* out) {

@@ -41,6 +41,9 @@ struct LaunchWork {
   unsigned int* counter;
   SegOut* out;
   int* inst_left;
+  int* plist;                  // re-track path list (capacity plist_cap)
+  unsigned int* plist_count;   // [1]
+  long long plist_cap;
 };
 
 // Hot, wave-uniform description of one segment (kernel arguments).
@@ -60,6 +63,8 @@ struct Seg {
   const double2* cht;          // [I][C+1][M]
   unsigned int* counter;
   const SegOut* out;           // device copy written by init_kernel
+  const int* plist;            // optional: track only these paths (count in *plist_count)
+  const unsigned int* plist_count;
 };
 
 struct Opts {
@@ -406,8 +411,13 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
       long long p;
       if (LN.persistent) {
         unsigned int q = 0;
-        if (want && g.gl == 0) q = atomicAdd(S.counter, 1u);
+        if (want && g.gl == 0) {
+          q = atomicAdd(S.counter, 1u);
+          // path-list launch (FP64 re-track of selected paths)
+          if (S.plist) q = q < *S.plist_count ? (unsigned int)S.plist[q] : 0xffffffffu;
+        }
         p = (long long)(unsigned int)lane_read((int)q, g.gbase);
+        if (p == 0xffffffffll) p = -1;
       } else {
         p = static_path;
       }
@@ -421,7 +431,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
           nsteps = 0;
           nnewt = 0;
           nref = 0;
-          fl = 0;
+          fl = S.plist ? HCX_FLAG_ESCALATED : 0u;
           start_path(MODE == 0);
           if (!LN.persistent) exhausted = true;
         }
@@ -595,6 +605,8 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
             if (O->inst_left) {
               const int left = atomicSub(&O->inst_left[inst], 1);
               if (left == 1 && O->inst_done) O->inst_done[inst] = wall_clock64();
+            } else if (O->inst_done && S.plist) {
+              atomicMax(&O->inst_done[inst], (unsigned long long)wall_clock64());
             }
           }
           status = 0;

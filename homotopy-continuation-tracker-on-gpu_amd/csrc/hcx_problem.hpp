@@ -23,6 +23,9 @@ struct hcx_problem {
   hcx::SegOut* d_out = nullptr;
   int* d_inst_left = nullptr;
   int inst_cap = 0;
+  int* d_plist = nullptr;             // FP64 re-track path list
+  unsigned int* d_plist_count = nullptr;
+  long long plist_cap = 0;
   // autotuned launch shape (hcx_autotune); used when the options leave it open
   bool tuned = false;
   int tuned_G = 0, tuned_spec = 1, tuned_wpc = 0;

@@ -99,6 +99,8 @@ struct Slot {
   hcx::SegOut* d_out = nullptr;
   int* d_inst_left = nullptr;
   unsigned long long* d_inst_done = nullptr;
+  int* d_plist = nullptr;  // FP64 re-track path list (mixed precision)
+  long long plist_cap = 0;
 };
 
 size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
@@ -130,21 +132,48 @@ struct hcx_server {
            align256((size_t)p->K * p->n * sizeof(hcx_complex)) + align256((size_t)p->K * sizeof(uint32_t));
   }
 
-  // Completes finished launches (mutex held).
-  void reap() {
+  // Results of one finished launch, unpacked outside the mutex.
+  struct Done {
+    Slot* slot;
+    std::vector<std::vector<hcx_complex>> roots;
+    std::vector<std::vector<uint32_t>> flags;
+    std::string err;
+  };
+
+  // Collects finished launches without the mutex (scheduler thread only).
+  void poll_slots(std::vector<Done>& out) {
     for (Slot& s : slots) {
       if (!s.busy) continue;
       const hipError_t q = hipEventQuery(s.done);
       if (q == hipErrorNotReady) continue;
-      if (q != hipSuccess && error.empty()) error = hipGetErrorString(q);
+      Done d;
+      d.slot = &s;
+      if (q != hipSuccess) d.err = hipGetErrorString(q);
       const hcx_problem* p = types[s.type];
-      const auto now = Clock::now();
+      const size_t nr = (size_t)p->K * p->n, nf = (size_t)p->K;
+      d.roots.resize(s.reqs.size());
+      d.flags.resize(s.reqs.size());
+      for (size_t i = 0; i < s.reqs.size(); ++i) {
+        d.roots[i].assign(s.h_roots + i * nr, s.h_roots + (i + 1) * nr);
+        d.flags[i].assign(s.h_flags + i * nf, s.h_flags + (i + 1) * nf);
+      }
+      out.push_back(std::move(d));
+    }
+  }
+
+  // Publishes finished launches (mutex held).
+  void reap(std::vector<Done>& done) {
+    const auto now = Clock::now();
+    for (Done& d : done) {
+      Slot& s = *d.slot;
+      if (!d.err.empty() && error.empty()) error = d.err;
+      const hcx_problem* p = types[s.type];
       for (size_t i = 0; i < s.reqs.size(); ++i) {
         auto it = reqs.find(s.reqs[i]);
         if (it == reqs.end()) continue;
         Request& r = it->second;
-        r.roots.assign(s.h_roots + i * (size_t)p->K * p->n, s.h_roots + (i + 1) * (size_t)p->K * p->n);
-        r.flags.assign(s.h_flags + i * (size_t)p->K, s.h_flags + (i + 1) * (size_t)p->K);
+        r.roots = std::move(d.roots[i]);
+        r.flags = std::move(d.flags[i]);
         r.refined = r.real = 0;
         for (uint32_t f : r.flags) {
           r.refined += (f & HCX_FLAG_REFINED) ? 1 : 0;
@@ -164,6 +193,7 @@ struct hcx_server {
       s.reqs.clear();
       s.busy = false;
     }
+    done.clear();
   }
 
   hcx_status ensure_staging(Slot& s, size_t params, size_t roots, size_t paths) {
@@ -188,16 +218,18 @@ struct hcx_server {
     return HCX_OK;
   }
 
-  // Launches one micro-batch on a free slot if any type has queued requests
-  // that fit the budget (mutex held).  Returns true if something launched.
-  bool launch_one() {
+  // Picks one micro-batch for a free slot if any type has queued requests
+  // that fit the budget (mutex held): pops the requests, reserves the arena
+  // block and stages the parameters in pinned memory.  Returns the slot or
+  // nullptr.  The HIP work is issued by issue() without the mutex.
+  Slot* pick() {
     Slot* slot = nullptr;
     for (Slot& s : slots)
       if (!s.busy) {
         slot = &s;
         break;
       }
-    if (!slot) return false;
+    if (!slot) return nullptr;
     const int nt = (int)types.size();
     for (int q = 0; q < nt; ++q) {
       const int t = (rr + q) % nt;
@@ -213,7 +245,7 @@ struct hcx_server {
       if (ensure_staging(s, (size_t)B * p->k, (size_t)B * p->K * p->n, (size_t)B * p->K) != HCX_OK) {
         arena.release(off, per * (size_t)B);
         if (error.empty()) error = "pinned staging allocation failed";
-        return false;
+        return nullptr;
       }
       s.type = t;
       s.off = off;
@@ -229,44 +261,54 @@ struct hcx_server {
         r.t_launch = now;
         s.reqs.push_back(id);
       }
-      // carve the arena block
-      char* base = d_arena + off;
-      hcx_complex* d_params = reinterpret_cast<hcx_complex*>(base);
-      base += align256((size_t)p->k * sizeof(hcx_complex)) * B;
-      hcx_complex* d_work = reinterpret_cast<hcx_complex*>(base);
-      base += align256((size_t)(p->C + 1) * 3 * sizeof(hcx_complex)) * B;
-      hcx_complex* d_roots = reinterpret_cast<hcx_complex*>(base);
-      base += align256((size_t)p->K * p->n * sizeof(hcx_complex)) * B;
-      uint32_t* d_flags = reinterpret_cast<uint32_t*>(base);
-      (void)hipMemcpyAsync(d_params, s.h_params, sizeof(hcx_complex) * (size_t)B * p->k, hipMemcpyHostToDevice,
-                           s.stream);
-      hcx_batch_desc b;
-      std::memset(&b, 0, sizeof(b));
-      b.num_instances = B;
-      b.d_target_params = d_params;
-      b.d_coeff_work = d_work;
-      hcx_result_desc r;
-      std::memset(&r, 0, sizeof(r));
-      r.d_roots = d_roots;
-      r.d_flags = d_flags;
-      r.d_inst_done = reinterpret_cast<uint64_t*>(s.d_inst_done);
-      hcx::LaunchWork w{s.d_counter, s.d_out, s.d_inst_left};
-      hcx_problem* pp = p;
-      const hcx_status ls = hcx::track_impl(1, &pp, &b, &r, &to, s.stream, 1, nullptr, &w);
-      if (ls != HCX_OK && error.empty()) error = hcx_last_error();
-      (void)hipMemcpyAsync(s.h_roots, d_roots, sizeof(hcx_complex) * (size_t)B * p->K * p->n, hipMemcpyDeviceToHost,
-                           s.stream);
-      (void)hipMemcpyAsync(s.h_flags, d_flags, sizeof(uint32_t) * (size_t)B * p->K, hipMemcpyDeviceToHost, s.stream);
-      (void)hipEventRecord(s.done, s.stream);
       s.busy = true;
       st.launches++;
       st.inflight_requests += B;
       st.inflight_paths += s.paths;
       st.peak_inflight_paths = std::max(st.peak_inflight_paths, st.inflight_paths);
       st.hbm_in_use += (int64_t)s.bytes;
-      return true;
+      return slot;
     }
-    return false;
+    return nullptr;
+  }
+
+  // Issues the copies, the launch and the completion event of a picked slot
+  // (no mutex: only the scheduler thread touches slots).
+  void issue(Slot& s) {
+    hcx_problem* p = types[s.type];
+    const int B = (int)s.reqs.size();
+    char* base = d_arena + s.off;
+    hcx_complex* d_params = reinterpret_cast<hcx_complex*>(base);
+    base += align256((size_t)p->k * sizeof(hcx_complex)) * B;
+    hcx_complex* d_work = reinterpret_cast<hcx_complex*>(base);
+    base += align256((size_t)(p->C + 1) * 3 * sizeof(hcx_complex)) * B;
+    hcx_complex* d_roots = reinterpret_cast<hcx_complex*>(base);
+    base += align256((size_t)p->K * p->n * sizeof(hcx_complex)) * B;
+    uint32_t* d_flags = reinterpret_cast<uint32_t*>(base);
+    (void)hipMemcpyAsync(d_params, s.h_params, sizeof(hcx_complex) * (size_t)B * p->k, hipMemcpyHostToDevice,
+                         s.stream);
+    hcx_batch_desc b;
+    std::memset(&b, 0, sizeof(b));
+    b.num_instances = B;
+    b.d_target_params = d_params;
+    b.d_coeff_work = d_work;
+    hcx_result_desc r;
+    std::memset(&r, 0, sizeof(r));
+    r.d_roots = d_roots;
+    r.d_flags = d_flags;
+    r.d_inst_done = reinterpret_cast<uint64_t*>(s.d_inst_done);
+    // d_counter holds 64 words: word 0 is the path queue, word 32 the re-track list length
+    hcx::LaunchWork w{s.d_counter, s.d_out, s.d_inst_left, s.d_plist, s.d_counter + 32, s.plist_cap};
+    hcx_problem* pp = p;
+    const hcx_status ls = hcx::track_impl(1, &pp, &b, &r, &to, s.stream, 1, nullptr, &w);
+    if (ls != HCX_OK) {
+      std::lock_guard<std::mutex> lk(mu);
+      if (error.empty()) error = hcx_last_error();
+    }
+    (void)hipMemcpyAsync(s.h_roots, d_roots, sizeof(hcx_complex) * (size_t)B * p->K * p->n, hipMemcpyDeviceToHost,
+                         s.stream);
+    (void)hipMemcpyAsync(s.h_flags, d_flags, sizeof(uint32_t) * (size_t)B * p->K, hipMemcpyDeviceToHost, s.stream);
+    (void)hipEventRecord(s.done, s.stream);
   }
 
   bool idle() const {
@@ -278,18 +320,27 @@ struct hcx_server {
   }
 
   void run() {
-    std::unique_lock<std::mutex> lk(mu);
+    std::vector<Done> done;
+    std::vector<Slot*> picked;
     while (true) {
-      reap();
-      bool any = false;
-      while (launch_one()) any = true;
-      if (any) cv.notify_all();
-      bool busy = false;
-      for (const Slot& s : slots) busy = busy || s.busy;
-      if (stop && idle()) break;
+      poll_slots(done);
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        reap(done);
+        picked.clear();
+        while (Slot* s = pick()) picked.push_back(s);
+        bool busy = false;
+        for (const Slot& s : slots) busy = busy || s.busy;
+        if (picked.empty()) {
+          if (stop && idle()) break;
+          cv.notify_all();
+          // wait for a submission, or poll the in-flight launches every 50 us
+          cv.wait_for(lk, std::chrono::microseconds(busy ? 50 : 1000));
+          continue;
+        }
+      }
+      for (Slot* s : picked) issue(*s);
       cv.notify_all();
-      // wait for a submission, or poll the in-flight launches every 50 us
-      cv.wait_for(lk, std::chrono::microseconds(busy ? 50 : 1000));
     }
     cv.notify_all();
   }
@@ -365,6 +416,19 @@ hcx_status hcx_server_add_type(hcx_server* s, hcx_problem* p, int* type_id) {
     hcx::set_error("hcx_server_add_type: one request of this type exceeds the HBM budget");
     return HCX_ERR_BUDGET;
   }
+  // re-track path lists hold every path of a full batch of the largest type
+  const long long need = (long long)s->so.max_batch_requests * p->K;
+  for (Slot& sl : s->slots) {
+    if (sl.plist_cap >= need) continue;
+    if (sl.d_plist) (void)hipFree(sl.d_plist);
+    sl.d_plist = nullptr;
+    sl.plist_cap = 0;
+    if (hipMalloc(&sl.d_plist, sizeof(int) * (size_t)need) != hipSuccess) {
+      hcx::set_error("hcx_server_add_type: cannot allocate the re-track path list");
+      return HCX_ERR_OUT_OF_MEMORY;
+    }
+    sl.plist_cap = need;
+  }
   s->types.push_back(p);
   s->queues.emplace_back();
   *type_id = (int)s->types.size() - 1;
@@ -426,6 +490,7 @@ hcx_status hcx_server_poll(hcx_server* s, uint64_t request_id, int* done, hcx_re
   Request& r = it->second;
   *done = r.done ? 1 : 0;
   if (!r.done) return HCX_OK;
+  if (!out && !h_roots && !h_flags) return HCX_OK;  // status query only
   const hcx_problem* p = s->types[r.type];
   if (out) {
     out->type_id = r.type;
@@ -520,6 +585,7 @@ hcx_status hcx_server_destroy(hcx_server* s) {
     if (sl.d_counter) (void)hipFree(sl.d_counter);
     if (sl.d_out) (void)hipFree(sl.d_out);
     if (sl.d_inst_left) (void)hipFree(sl.d_inst_left);
+    if (sl.d_plist) (void)hipFree(sl.d_plist);
     if (sl.d_inst_done) (void)hipFree(sl.d_inst_done);
   }
   if (s->d_arena) (void)hipFree(s->d_arena);

@@ -127,6 +127,37 @@ __global__ void collect_kernel(int n, int K, const double2* __restrict__ roots, 
   atomicAdd(&counts[4], c_uni);
 }
 
+// Marks every refined path whose root equals another refined root of the
+// same instance (|dRe|, |dIm| < tol in every coordinate): a path that jumped
+// onto another path's root shares that root.  The marked paths are appended
+// to plist for an FP64 re-track.  One block per instance.
+__global__ void dup_mark_kernel(int n, int K, const double2* __restrict__ roots, const uint32_t* __restrict__ flags,
+                                double tol, int* __restrict__ plist, unsigned int* __restrict__ count) {
+  const int inst = blockIdx.x;
+  const size_t base = (size_t)inst * K;
+  for (int j = threadIdx.x; j < K; j += blockDim.x) {
+    if (!(flags[base + j] & HCX_FLAG_REFINED)) continue;
+    const double2* xj = roots + (base + j) * n;
+    bool dup = false;
+    for (int i = 0; i < K && !dup; ++i) {
+      if (i == j || !(flags[base + i] & HCX_FLAG_REFINED)) continue;
+      const double2* xi = roots + (base + i) * n;
+      bool same = true;
+      for (int v = 0; v < n && same; ++v) same = fabs(xi[v].x - xj[v].x) < tol && fabs(xi[v].y - xj[v].y) < tol;
+      dup = same;
+    }
+    if (dup) {
+      const unsigned int q = atomicAdd(count, 1u);
+      plist[q] = (int)(base + j);
+    }
+  }
+}
+
+__global__ void zero_u32_kernel(unsigned int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 __global__ void zero_u64_kernel(unsigned long long* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -336,6 +367,17 @@ hcx_status finish_plan(const Launch& LN, Plan& pl) {
 // Default lanes per path: n (several paths per wave, G need not divide 64).
 int default_lanes(int n) { return n; }
 
+hcx_status ensure_plist_cap(hcx_problem* p, long long P) {
+  if (!p->d_plist_count) HIP_TRY(hipMalloc(&p->d_plist_count, sizeof(unsigned int)));
+  if (p->plist_cap >= P) return HCX_OK;
+  if (p->d_plist) (void)hipFree(p->d_plist);
+  p->d_plist = nullptr;
+  p->plist_cap = 0;
+  HIP_TRY(hipMalloc(&p->d_plist, sizeof(int) * (size_t)std::max<long long>(P, 1)));
+  p->plist_cap = std::max<long long>(P, 1);
+  return HCX_OK;
+}
+
 hcx_status ensure_inst_cap(hcx_problem* p, int I) {
   if (p->inst_cap >= I) return HCX_OK;
   if (p->d_inst_left) (void)hipFree(p->d_inst_left);
@@ -538,6 +580,8 @@ hcx_status hcx_problem_destroy(hcx_problem* p) {
   if (p->d_counter) (void)hipFree(p->d_counter);
   if (p->d_out) (void)hipFree(p->d_out);
   if (p->d_inst_left) (void)hipFree(p->d_inst_left);
+  if (p->d_plist) (void)hipFree(p->d_plist);
+  if (p->d_plist_count) (void)hipFree(p->d_plist_count);
   delete p;
   return HCX_OK;
 }
@@ -727,6 +771,59 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(pl.fn, dim3((unsigned)blocks), dim3(64), pl.lds, stream, LN);
   HIP_TRY(hipGetLastError());
+
+  // Mixed precision: re-track in FP64 every path whose root duplicates another
+  // root of its instance (a jump onto another path), so the refined roots and
+  // counts are those of the FP64 tracker.
+  if (persistent && o->precision == HCX_PREC_MIXED && o->retrack_duplicates) {
+    Launch L2 = LN;
+    OutPack OP2 = OP;
+    for (int s = 0; s < nseg; ++s) {
+      hcx_problem* p = probs[s];
+      Seg& S = L2.seg[s];
+      int* plist;
+      unsigned int* pcount;
+      if (work) {
+        if (work[s].plist_cap < S.P || !work[s].plist || !work[s].plist_count) {
+          set_error("track: launch workspace path list too small (%lld < %lld)", work[s].plist_cap, S.P);
+          return HCX_ERR_INVALID_ARG;
+        }
+        plist = work[s].plist;
+        pcount = work[s].plist_count;
+      } else {
+        st = ensure_plist_cap(p, S.P);
+        if (st != HCX_OK) return st;
+        plist = p->d_plist;
+        pcount = p->d_plist_count;
+      }
+      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, pcount, 1);
+      if (S.I > 0)
+        hipLaunchKernelGGL(dup_mark_kernel, dim3(S.I), dim3(256), 0, stream, S.n, S.K,
+                           reinterpret_cast<const double2*>(results[s].d_roots), results[s].d_flags, o->unique_tol,
+                           plist, pcount);
+      S.plist = plist;
+      S.plist_count = pcount;
+      OP2.out[s].inst_left = nullptr;
+    }
+    HIP_TRY(hipGetLastError());
+    Plan p2;
+    st = choose_kernel(nseg, ns, Gs, max_deg, HCX_PREC_FP64, specialize, p2);
+    if (st != HCX_OK) return st;
+    for (int s = 0; s < nseg; ++s) L2.seg[s].lds_group = lds_group_bytes(p2.nmax, probs[s]->C, HCX_PREC_FP64, probs[s]->M);
+    st = finish_plan(L2, p2);
+    if (st != HCX_OK) return st;
+    // a small persistent grid: the list holds few paths; idle waves exit at once
+    long long b2 = 0;
+    for (int s = 0; s < nseg; ++s) {
+      L2.first_block[s] = (int)b2;
+      const int gpw = 64 / L2.seg[s].G;
+      b2 += std::max<long long>(1, std::min<long long>((L2.seg[s].P + gpw - 1) / gpw, g_cu / nseg + 1));
+    }
+    L2.first_block[nseg] = (int)b2;
+    hipLaunchKernelGGL(init_kernel, dim3(1, nseg), dim3(64), 0, stream, L2, OP2, (unsigned long long*)nullptr);
+    hipLaunchKernelGGL(p2.fn, dim3((unsigned)b2), dim3(64), p2.lds, stream, L2);
+    HIP_TRY(hipGetLastError());
+  }
   return HCX_OK;
 }
 

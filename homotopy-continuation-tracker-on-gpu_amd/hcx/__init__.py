@@ -409,8 +409,16 @@ class Server:
         _check(lib().hcx_server_submit(self._h, type_id, _ptr(p), C.byref(rid)), "hcx_server_submit")
         return rid.value
 
+    def done(self, request_id):
+        d = C.c_int()
+        _check(lib().hcx_server_poll(self._h, C.c_uint64(request_id), C.byref(d), None, None, None),
+               "hcx_server_poll")
+        return bool(d.value)
+
     def result(self, request_id, type_id):
         """Blocking-free fetch: returns None until done, else (info dict, roots, flags)."""
+        if not self.done(request_id):
+            return None
         prob = self.types[type_id]
         roots = np.empty((prob.num_tracks, prob.n), dtype=np.complex128)
         flags = np.empty(prob.num_tracks, dtype=np.uint32)

@@ -17,7 +17,7 @@ class TrackOpts(C.Structure):
         ("corr_tol", C.c_double), ("inf_tol", C.c_double),
         ("refine_iters", C.c_int), ("refine_tol", C.c_double), ("real_tol", C.c_double),
         ("unique_tol", C.c_double), ("precision", C.c_int), ("lanes_per_path", C.c_int),
-        ("waves_per_cu", C.c_int), ("specialize", C.c_int),
+        ("waves_per_cu", C.c_int), ("specialize", C.c_int), ("retrack_duplicates", C.c_int),
     ]
 
 

@@ -67,13 +67,17 @@ typedef struct {
   double corr_tol;        /* Newton success: |dx|^2 < corr_tol * |x|^2, 1e-6 (:214) */
   double inf_tol;         /* infinity failure: |x|^2 > inf_tol, 1e14 (:215) */
   int refine_iters;       /* FP64 Newton iterations on H(x,1) after tracking (0 = off) */
-  double refine_tol;      /* refinement success: |dx| <= refine_tol * max(1,|x|) */
+  double refine_tol;      /* refinement success: |dx| <= refine_tol * max(1,|x|) (1e-10; Newton converges
+                             quadratically, so the accepted root is far more accurate than the last step) */
   double real_tol;        /* root is real if all |Im x_i| <= real_tol (definitions.hpp:25-26) */
   double unique_tol;      /* duplicate if all |dRe|,|dIm| < unique_tol (definitions.hpp:24) */
   int precision;          /* hcx_precision */
   int lanes_per_path;     /* lanes of a wave64 per path (G); 0 = autotuned / n */
   int waves_per_cu;       /* persistent grid size per CU; 0 = from occupancy */
   int specialize;         /* 1: size-specialised kernel instance when one exists (n = G); 0: generic */
+  int retrack_duplicates; /* mixed precision: re-track in FP64 every path whose refined root duplicates
+                             another root of its instance (path jumping), so roots and counts are the
+                             FP64 tracker's (1, default) */
 } hcx_track_opts;
 
 void hcx_track_opts_default(hcx_track_opts* o);
@@ -262,7 +266,8 @@ hcx_status hcx_server_add_type(hcx_server* s, hcx_problem* p, int* type_id);
  * Thread-safe; returns immediately. */
 hcx_status hcx_server_submit(hcx_server* s, int type_id, const hcx_complex* h_target_params, uint64_t* request_id);
 /* *done = 1 once the request finished; then fills *out and (if non-NULL)
- * h_roots [K][n] and h_flags [K], and forgets the request. */
+ * h_roots [K][n] and h_flags [K], and forgets the request.  With out, h_roots
+ * and h_flags all NULL it only reports *done and keeps the request. */
 hcx_status hcx_server_poll(hcx_server* s, uint64_t request_id, int* done, hcx_request_result* out,
                            hcx_complex* h_roots, uint32_t* h_flags);
 /* Waits until every submitted request has finished (or the timeout). */

@@ -29,7 +29,7 @@ class Opts(C.Structure):
 
 
 DEFAULTS = dict(max_steps=200, max_corr=3, incr_steps=3, dt0=0.05, end_zone=0.05, t_tol=1e-7, dt_min=1e-7,
-                corr_tol=1e-6, inf_tol=1e14, refine_iters=8, refine_tol=1e-13, real_tol=1e-8)
+                corr_tol=1e-6, inf_tol=1e14, refine_iters=8, refine_tol=1e-10, real_tol=1e-8)
 
 _lib = None
 
