@@ -69,6 +69,7 @@ struct Seg {
 
 struct Opts {
   int max_steps, max_corr, incr, refine_iters;
+  int newton32;      // mixed: FP32 corrector before the end zone
   double dt0, t_tol, dt_min, corr_tol, inf_tol, refine_tol, real_tol;
   double ez_enter;   // enter the end zone when |1 - t| <= end_zone + 1e-7
   double ez_land;    // before the end zone, steps land on 1 - end_zone
@@ -352,10 +353,11 @@ __device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, cons
   return acc;
 }
 
-// Lane-group path loop.  MODE 0: everything FP64.  MODE 1 (mixed): FP32
-// predictor and Jacobian, FP32 corrector before the end zone, FP64 corrector
-// in the end zone, FP64 refinement; a path that ends without a refined root
-// is re-tracked from t = 0 entirely in FP64 by the same group (escalation).
+// Lane-group path loop.  MODE 0: everything FP64.  MODE 1 (mixed): FP32 RK4
+// predictor (Jacobian, dH/dt and LU of the four stages), FP64 Newton
+// corrector (FP32 before the end zone when o.newton32), FP64 refinement; a
+// path that ends without a refined root is re-tracked from t = 0 entirely in
+// FP64 by the same group (escalation).
 template <int NMAX, int DEG, int MODE, bool EXACT>
 __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned char* smem, int lane) {
   // local copies: the fields are wave-uniform and live in SGPRs (scalar
@@ -489,7 +491,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
       if (MODE == 0) {
         newton_pass<NMAX, DEG, double>(S, o, g, L, stepping, row, x, nnewt, ok, inf);
       } else {
-        const bool run32 = stepping && !ez && !esc, run64 = stepping && (ez || esc);
+        const bool run32 = stepping && !ez && !esc && o.newton32, run64 = stepping && !run32;
         bool ok32 = false, inf32 = false, ok64 = false, inf64 = false;
         if (__any(run32)) newton_pass<NMAX, DEG, float>(S, o, g, L, run32, row, x, nnewt, ok32, inf32);
         if (__any(run64)) {

@@ -23,6 +23,9 @@ struct hcx_problem {
   hcx::SegOut* d_out = nullptr;
   int* d_inst_left = nullptr;
   int inst_cap = 0;
+  double2* d_cmap_scalar = nullptr;   // parameter-polynomial coefficient map [C][cmap_T]
+  int* d_cmap_params = nullptr;       // [C][cmap_T][cmap_D]
+  int cmap_T = 0, cmap_D = 0;
   int* d_plist = nullptr;             // FP64 re-track path list
   unsigned int* d_plist_count = nullptr;
   long long plist_cap = 0;

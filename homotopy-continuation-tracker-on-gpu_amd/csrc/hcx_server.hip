@@ -128,7 +128,8 @@ struct hcx_server {
   std::string error;
 
   size_t request_bytes(const hcx_problem* p) const {
-    return align256((size_t)p->k * sizeof(hcx_complex)) + align256((size_t)(p->C + 1) * 3 * sizeof(hcx_complex)) +
+    return align256((size_t)p->k * sizeof(hcx_complex)) +
+           align256((size_t)(p->C + 1) * (2 * p->M + 1) * sizeof(hcx_complex)) +
            align256((size_t)p->K * p->n * sizeof(hcx_complex)) + align256((size_t)p->K * sizeof(uint32_t));
   }
 
@@ -281,7 +282,7 @@ struct hcx_server {
     hcx_complex* d_params = reinterpret_cast<hcx_complex*>(base);
     base += align256((size_t)p->k * sizeof(hcx_complex)) * B;
     hcx_complex* d_work = reinterpret_cast<hcx_complex*>(base);
-    base += align256((size_t)(p->C + 1) * 3 * sizeof(hcx_complex)) * B;
+    base += align256((size_t)(p->C + 1) * (2 * p->M + 1) * sizeof(hcx_complex)) * B;
     hcx_complex* d_roots = reinterpret_cast<hcx_complex*>(base);
     base += align256((size_t)p->K * p->n * sizeof(hcx_complex)) * B;
     uint32_t* d_flags = reinterpret_cast<uint32_t*>(base);
@@ -403,8 +404,8 @@ hcx_status hcx_server_add_type(hcx_server* s, hcx_problem* p, int* type_id) {
     hcx::set_error("hcx_server_add_type: null argument");
     return HCX_ERR_INVALID_ARG;
   }
-  if (p->k < 1 || p->M != 1 || !p->d_affine) {
-    hcx::set_error("hcx_server_add_type: the problem needs a parameter-affine coefficient map");
+  if (p->k < 1 || (!p->d_cmap_scalar && (p->M != 1 || !p->d_affine))) {
+    hcx::set_error("hcx_server_add_type: the problem needs a parameter coefficient map (affine or polynomial)");
     return HCX_ERR_INVALID_ARG;
   }
   std::lock_guard<std::mutex> lk(s->mu);

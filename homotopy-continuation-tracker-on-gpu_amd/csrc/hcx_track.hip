@@ -89,6 +89,75 @@ __global__ void p2c_affine_kernel(const double2* __restrict__ A, int C, int k, c
   hti[0] = make_double2(r1 - r0, i1 - i0);
 }
 
+// Parameter-polynomial P2C on the device: for instance i and coefficient j,
+// c_j(p(t)) with p(t) = p0 + t (p1 - p0) and
+// c_j(p) = sum_u scalar[j][u] prod_d p[params[j][u][d]] (p[k] = 1) as its
+// t^0..t^M coefficients Hx[i][j][0..M] and those of dc_j/dt, Ht[i][j][0..M-1]
+// (the role of the generated p2c-<p>.h, Data_Reader.cpp:59-89, with the
+// coefficient-polynomial layout of createUnivPoly4Coeffs.m; entry C = {1, 0..}).
+// Each term is a product of linear factors (q_a + t d_a), expanded by
+// repeated polynomial multiplication.
+constexpr int kMaxOrderT = 8;
+
+__global__ void p2c_poly_kernel(const double2* __restrict__ scal, const int* __restrict__ prm, int C, int T, int D,
+                                int k, int M, const double2* __restrict__ p0, int p0_stride,
+                                const double2* __restrict__ p1, int I, double2* __restrict__ hx,
+                                double2* __restrict__ ht) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j > C || i >= I) return;
+  double2* hxi = hx + ((size_t)i * (C + 1) + j) * (M + 1);
+  double2* hti = ht + ((size_t)i * (C + 1) + j) * M;
+  double cr[kMaxOrderT + 1], ci[kMaxOrderT + 1];
+#pragma unroll
+  for (int m = 0; m <= kMaxOrderT; ++m) cr[m] = ci[m] = 0.0;
+  if (j == C) {
+    cr[0] = 1.0;
+  } else {
+    const double2* q0 = p0 + (size_t)i * p0_stride;
+    const double2* q1 = p1 + (size_t)i * k;
+    for (int u = 0; u < T; ++u) {
+      const double2 sc = scal[(size_t)j * T + u];
+      if (sc.x == 0.0 && sc.y == 0.0) continue;
+      double tr[kMaxOrderT + 1], ti[kMaxOrderT + 1];
+#pragma unroll
+      for (int m = 0; m <= kMaxOrderT; ++m) tr[m] = ti[m] = 0.0;
+      tr[0] = sc.x;
+      ti[0] = sc.y;
+      int deg = 0;
+      for (int d = 0; d < D; ++d) {
+        const int a = prm[((size_t)j * T + u) * D + d];
+        if (a >= k) continue;  // the constant-1 padding factor
+        const double2 qa = q0[a], pa = q1[a];
+        const double dr = pa.x - qa.x, di = pa.y - qa.y;
+        // (tr, ti) *= (qa + t d): new[m] = old[m] qa + old[m-1] d
+#pragma unroll
+        for (int m = kMaxOrderT; m >= 0; --m) {
+          if (m > deg + 1) continue;
+          double nr = tr[m] * qa.x - ti[m] * qa.y, ni = tr[m] * qa.y + ti[m] * qa.x;
+          if (m > 0) {
+            nr += tr[m - 1] * dr - ti[m - 1] * di;
+            ni += tr[m - 1] * di + ti[m - 1] * dr;
+          }
+          tr[m] = nr;
+          ti[m] = ni;
+        }
+        ++deg;
+      }
+#pragma unroll
+      for (int m = 0; m <= kMaxOrderT; ++m) {
+        cr[m] += tr[m];
+        ci[m] += ti[m];
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m <= kMaxOrderT; ++m) {
+    if (m <= M) hxi[m] = make_double2(cr[m], ci[m]);
+    if (m < M) hti[m] = make_double2((m + 1) * cr[m + 1], (m + 1) * ci[m + 1]);
+  }
+}
+
 // Solution collection on the device (Evaluations.cpp:88-176, first-occurrence
 // uniqueness): one block per instance.
 __global__ void collect_kernel(int n, int K, const double2* __restrict__ roots, uint32_t* __restrict__ flags,
@@ -267,6 +336,7 @@ Opts to_opts(const hcx_track_opts* o) {
   r.inf_tol = o->inf_tol;
   r.refine_tol = o->refine_tol;
   r.real_tol = o->real_tol;
+  r.newton32 = o->fp32_corrector ? 1 : 0;
   return r;
 }
 
@@ -414,10 +484,24 @@ hcx_status hcx_problem_create(const hcx_problem_desc* d, hcx_problem** out) {
   if (n < 1 || n > kMaxN || C < 1 || M < 1 || d->hx_terms < 1 || d->ht_terms < 1 || d->hx_parts < 3 ||
       d->ht_parts < 3 || d->hx_parts > 10 || d->ht_parts > 10 || !d->h_hx_idx || !d->h_ht_idx ||
       d->num_tracks < 1 || !d->h_start_sols || d->num_params < 0 ||
-      (d->num_params > 0 && (!d->h_coeff_affine || !d->h_start_params))) {
+      (d->num_params > 0 && (!d->h_start_params || (!d->h_coeff_affine) == (!d->h_cmap_scalar)))) {
     set_error("hcx_problem_create: invalid descriptor (n=%d C=%d M=%d hx %dx%d ht %dx%d K=%d k=%d)", n, C, M,
               d->hx_terms, d->hx_parts, d->ht_terms, d->ht_parts, d->num_tracks, d->num_params);
     return HCX_ERR_INVALID_ARG;
+  }
+  if (d->h_cmap_scalar) {
+    if (!d->h_cmap_params || d->cmap_terms < 1 || d->cmap_deg < 1 || d->cmap_deg > M || M > kMaxOrderT) {
+      set_error("hcx_problem_create: invalid coefficient map (terms %d, degree %d, M %d; need 1 <= degree <= M <= %d)",
+                d->cmap_terms, d->cmap_deg, M, kMaxOrderT);
+      return HCX_ERR_INVALID_ARG;
+    }
+    const size_t np = (size_t)C * d->cmap_terms * d->cmap_deg;
+    for (size_t q = 0; q < np; ++q)
+      if (d->h_cmap_params[q] < 0 || d->h_cmap_params[q] > d->num_params) {
+        set_error("hcx_problem_create: coefficient map parameter index %d out of range 0..%d", d->h_cmap_params[q],
+                  d->num_params);
+        return HCX_ERR_INVALID_ARG;
+      }
   }
   if (hcx_device_count() < 1) {
     set_error("hcx_problem_create: no HIP device available");
@@ -517,13 +601,27 @@ hcx_status hcx_problem_create(const hcx_problem_desc* d, hcx_problem** out) {
   if ((e = hipMemcpy(p->d_start, d->h_start_sols, (size_t)p->K * n * sizeof(double2), hipMemcpyHostToDevice)) !=
       hipSuccess)
     return fail(e, "copy start");
-  if (p->k > 0) {
+  if (p->k > 0 && d->h_coeff_affine) {
     if ((e = hipMalloc(&p->d_affine, (size_t)C * (1 + p->k) * sizeof(double2))) != hipSuccess)
       return fail(e, "hipMalloc affine");
-    if ((e = hipMalloc(&p->d_p0, (size_t)p->k * sizeof(double2))) != hipSuccess) return fail(e, "hipMalloc p0");
     if ((e = hipMemcpy(p->d_affine, d->h_coeff_affine, (size_t)C * (1 + p->k) * sizeof(double2),
                        hipMemcpyHostToDevice)) != hipSuccess)
       return fail(e, "copy affine");
+  }
+  if (p->k > 0 && d->h_cmap_scalar) {
+    p->cmap_T = d->cmap_terms;
+    p->cmap_D = d->cmap_deg;
+    const size_t ns = (size_t)C * p->cmap_T, np = ns * p->cmap_D;
+    if ((e = hipMalloc(&p->d_cmap_scalar, ns * sizeof(double2))) != hipSuccess) return fail(e, "hipMalloc cmap");
+    if ((e = hipMalloc(&p->d_cmap_params, np * sizeof(int))) != hipSuccess) return fail(e, "hipMalloc cmap");
+    if ((e = hipMemcpy(p->d_cmap_scalar, d->h_cmap_scalar, ns * sizeof(double2), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+      return fail(e, "copy cmap");
+    if ((e = hipMemcpy(p->d_cmap_params, d->h_cmap_params, np * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(e, "copy cmap");
+  }
+  if (p->k > 0) {
+    if ((e = hipMalloc(&p->d_p0, (size_t)p->k * sizeof(double2))) != hipSuccess) return fail(e, "hipMalloc p0");
     if ((e = hipMemcpy(p->d_p0, d->h_start_params, (size_t)p->k * sizeof(double2), hipMemcpyHostToDevice)) !=
         hipSuccess)
       return fail(e, "copy p0");
@@ -577,6 +675,8 @@ hcx_status hcx_problem_destroy(hcx_problem* p) {
   if (p->d_start) (void)hipFree(p->d_start);
   if (p->d_affine) (void)hipFree(p->d_affine);
   if (p->d_p0) (void)hipFree(p->d_p0);
+  if (p->d_cmap_scalar) (void)hipFree(p->d_cmap_scalar);
+  if (p->d_cmap_params) (void)hipFree(p->d_cmap_params);
   if (p->d_counter) (void)hipFree(p->d_counter);
   if (p->d_out) (void)hipFree(p->d_out);
   if (p->d_inst_left) (void)hipFree(p->d_inst_left);
@@ -685,19 +785,24 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
       S.chx = reinterpret_cast<const double2*>(b.d_coeffs_hx);
       S.cht = reinterpret_cast<const double2*>(b.d_coeffs_ht);
     } else {
-      if (!b.d_target_params || !b.d_coeff_work || p->k < 1 || p->M != 1) {
-        set_error("track: segment %d: needs coefficient tables or target params + affine map + workspace", s);
+      if (!b.d_target_params || !b.d_coeff_work || p->k < 1 || (!p->d_cmap_scalar && (!p->d_affine || p->M != 1))) {
+        set_error("track: segment %d: needs coefficient tables or target params + a coefficient map + workspace", s);
         return HCX_ERR_INVALID_ARG;
       }
       double2* hx = reinterpret_cast<double2*>(b.d_coeff_work);
-      double2* ht = hx + (size_t)b.num_instances * (p->C + 1) * 2;
+      double2* ht = hx + (size_t)b.num_instances * (p->C + 1) * (p->M + 1);
       if (b.num_instances > 0) {
         const double2* p0 = b.d_start_params ? reinterpret_cast<const double2*>(b.d_start_params) : p->d_p0;
         const int p0_stride = b.d_start_params ? p->k : 0;
         dim3 grid((p->C + 1 + 127) / 128, b.num_instances);
-        hipLaunchKernelGGL(p2c_affine_kernel, grid, dim3(128), 0, stream, reinterpret_cast<const double2*>(p->d_affine),
-                           p->C, p->k, p0, p0_stride, reinterpret_cast<const double2*>(b.d_target_params),
-                           b.num_instances, hx, ht);
+        if (p->d_cmap_scalar)
+          hipLaunchKernelGGL(p2c_poly_kernel, grid, dim3(128), 0, stream, p->d_cmap_scalar, p->d_cmap_params, p->C,
+                             p->cmap_T, p->cmap_D, p->k, p->M, p0, p0_stride,
+                             reinterpret_cast<const double2*>(b.d_target_params), b.num_instances, hx, ht);
+        else
+          hipLaunchKernelGGL(p2c_affine_kernel, grid, dim3(128), 0, stream,
+                             reinterpret_cast<const double2*>(p->d_affine), p->C, p->k, p0, p0_stride,
+                             reinterpret_cast<const double2*>(b.d_target_params), b.num_instances, hx, ht);
       }
       S.chx = hx;
       S.cht = ht;

@@ -175,9 +175,11 @@ class System:
 
 
 class Problem:
-    """Device-resident problem type: tables + start solutions (+ affine P2C map)."""
+    """Device-resident problem type: tables + start solutions (+ a coefficient map:
+    affine, or parameter-polynomial given as cmap = (scalar [C][T], params [C][T][D]))."""
 
-    def __init__(self, n, num_coeffs, hx_idx, ht_idx, start_sols, max_order_t=1, affine=None, p0=None):
+    def __init__(self, n, num_coeffs, hx_idx, ht_idx, start_sols, max_order_t=1, affine=None, p0=None,
+                 cmap=None):
         hx_idx = np.ascontiguousarray(hx_idx, dtype=np.int32)
         ht_idx = np.ascontiguousarray(ht_idx, dtype=np.int32)
         self.start_sols = np.ascontiguousarray(start_sols, dtype=np.complex128).reshape(-1, n)
@@ -198,6 +200,16 @@ class Problem:
             self._keep += [affine, p0]
             d.num_params = affine.shape[1] - 1
             d.h_coeff_affine = affine.ctypes.data
+            d.h_start_params = p0.ctypes.data
+        elif cmap is not None:
+            scal = np.ascontiguousarray(cmap[0], dtype=np.complex128)
+            prm = np.ascontiguousarray(cmap[1], dtype=np.int32)
+            p0 = np.ascontiguousarray(p0, dtype=np.complex128)
+            self._keep += [scal, prm, p0]
+            d.num_params = p0.shape[0]
+            d.cmap_terms, d.cmap_deg = prm.shape[1], prm.shape[2]
+            d.h_cmap_scalar = scal.ctypes.data
+            d.h_cmap_params = prm.ctypes.data_as(C.POINTER(C.c_int))
             d.h_start_params = p0.ctypes.data
         self.n = n
         self.num_coeffs = num_coeffs
@@ -302,8 +314,8 @@ class Batch:
         self.start_params = dev(start_params)
         self.work = None
         if self.coeffs_hx is None:
-            self.work = torch.zeros(max(num_instances, 1) * (problem.num_coeffs + 1) * 3, dtype=torch.complex128,
-                                    device=device)
+            self.work = torch.zeros(max(num_instances, 1) * (problem.num_coeffs + 1) * (2 * problem.max_order_t + 1),
+                                    dtype=torch.complex128, device=device)
 
     def desc(self):
         b = BatchDesc()

@@ -17,7 +17,7 @@ class TrackOpts(C.Structure):
         ("corr_tol", C.c_double), ("inf_tol", C.c_double),
         ("refine_iters", C.c_int), ("refine_tol", C.c_double), ("real_tol", C.c_double),
         ("unique_tol", C.c_double), ("precision", C.c_int), ("lanes_per_path", C.c_int),
-        ("waves_per_cu", C.c_int), ("specialize", C.c_int), ("retrack_duplicates", C.c_int),
+        ("waves_per_cu", C.c_int), ("specialize", C.c_int), ("retrack_duplicates", C.c_int), ("fp32_corrector", C.c_int),
     ]
 
 
@@ -28,6 +28,8 @@ class ProblemDesc(C.Structure):
         ("h_hx_idx", C.POINTER(C.c_int)), ("h_ht_idx", C.POINTER(C.c_int)),
         ("num_tracks", C.c_int), ("h_start_sols", C.c_void_p),
         ("num_params", C.c_int), ("h_coeff_affine", C.c_void_p), ("h_start_params", C.c_void_p),
+        ("cmap_terms", C.c_int), ("cmap_deg", C.c_int), ("h_cmap_scalar", C.c_void_p),
+        ("h_cmap_params", C.POINTER(C.c_int)),
     ]
 
 

@@ -53,7 +53,8 @@ int hcx_device_count(void);
 /* ------------------------------------------------------------------------ */
 typedef enum {
   HCX_PREC_FP64 = 0,   /* everything complex FP64 */
-  HCX_PREC_MIXED = 1   /* FP32 predictor + Jacobian; FP64 corrector for t > 1 - end_zone; FP64 refinement */
+  HCX_PREC_MIXED = 1   /* FP32 RK4 predictor (Jacobian, dH/dt, LU); FP64 Newton corrector (FP32 before the
+                          end zone when fp32_corrector = 1); FP64 refinement */
 } hcx_precision;
 
 typedef struct {
@@ -78,6 +79,8 @@ typedef struct {
   int retrack_duplicates; /* mixed precision: re-track in FP64 every path whose refined root duplicates
                              another root of its instance (path jumping), so roots and counts are the
                              FP64 tracker's (1, default) */
+  int fp32_corrector;     /* mixed precision: 1 = Newton corrector in FP32 before the end zone (FP64 in it);
+                             0 (default) = FP64 corrector at every t, so step acceptance is decided in FP64 */
 } hcx_track_opts;
 
 void hcx_track_opts_default(hcx_track_opts* o);
@@ -103,6 +106,18 @@ typedef struct {
   int num_params;                  /* k (0 = no map) */
   const hcx_complex* h_coeff_affine; /* [C][1+k] or NULL */
   const hcx_complex* h_start_params; /* [k] start parameters p0 or NULL */
+  /* Optional parameter-polynomial coefficient map, the alternative to the
+   * affine map for coefficients that are polynomials in the parameters (the
+   * reference's P2C problems, whose generated p2c-<p>.h headers play this role,
+   * Data_Reader.cpp:59-89; and the PH type, coefficient = p_a * p_b):
+   *   c_j(p) = sum_{u < cmap_terms} cmap_scalar[j][u] * prod_{d < cmap_deg} p[cmap_params[j][u][d]]
+   * with p[k] = 1 as the padding factor.  On the straight-line parameter path
+   * p(t) = p0 + t (p1 - p0) (createUnivPoly4Coeffs.m) each c_j is a polynomial
+   * in t of degree <= cmap_deg <= max_order_t, built on the device per instance.
+   * Needs num_params = k and h_start_params; h_coeff_affine must then be NULL. */
+  int cmap_terms, cmap_deg;
+  const hcx_complex* h_cmap_scalar;  /* [C][cmap_terms] (0 = padding term) */
+  const int* h_cmap_params;          /* [C][cmap_terms][cmap_deg], entries in 0..k */
 } hcx_problem_desc;
 
 typedef struct hcx_problem hcx_problem;
@@ -140,10 +155,11 @@ typedef struct {
    * Data_Reader.cpp:59-89):  Hx[I][C+1][M+1] and Ht[I][C+1][M] ...           */
   const hcx_complex* d_coeffs_hx;
   const hcx_complex* d_coeffs_ht;
-  /* ... or per-instance target parameters p1 [I][k] (needs the affine map);
-   * p(t) = (1-t) p0 + t p1 (M = 1).  d_start_params [I][k] may be NULL
-   * (problem's p0 for every instance).  When d_coeffs_hx is non-NULL these
-   * are ignored.  d_coeff_work must then hold I*(C+1)*3 hcx_complex.        */
+  /* ... or per-instance target parameters p1 [I][k] (needs the affine or the
+   * parameter-polynomial coefficient map); p(t) = (1-t) p0 + t p1.
+   * d_start_params [I][k] may be NULL (problem's p0 for every instance).  When
+   * d_coeffs_hx is non-NULL these are ignored.  d_coeff_work must then hold
+   * I*(C+1)*(2M+1) hcx_complex (M = max_order_t; 3 per coefficient at M = 1). */
   const hcx_complex* d_target_params;
   const hcx_complex* d_start_params;
   hcx_complex* d_coeff_work;
