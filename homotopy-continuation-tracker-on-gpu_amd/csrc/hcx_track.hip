@@ -434,8 +434,16 @@ hcx_status finish_plan(const Launch& LN, Plan& pl) {
   return HCX_OK;
 }
 
-// Default lanes per path: n (several paths per wave, G need not divide 64).
-int default_lanes(int n) { return n; }
+// Default lanes per path: n (several paths per wave, G need not divide 64),
+// widened while the wave's groups would need more than 64 KiB of LDS (wide
+// coefficient tables), so that at least two waves share a CU.
+int default_lanes(int n, int C, int M, int mode) {
+  const int ni = round_up_idx(kNmaxList, 3, n);
+  const int lg = lds_group_bytes(ni < 0 ? n : kNmaxList[ni], C, mode, M);
+  int G = n;
+  while (G < 64 && (size_t)((64 + G - 1) / G) * (size_t)lg > 65536) ++G;
+  return G;
+}
 
 hcx_status ensure_plist_cap(hcx_problem* p, long long P) {
   if (!p->d_plist_count) HIP_TRY(hipMalloc(&p->d_plist_count, sizeof(unsigned int)));
@@ -750,7 +758,7 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
     ns[s] = probs[s]->n;
     if (o->lanes_per_path > 0) Gs[s] = o->lanes_per_path;
     else if (probs[s]->tuned) Gs[s] = probs[s]->tuned_G;
-    else Gs[s] = default_lanes(probs[s]->n);
+    else Gs[s] = default_lanes(probs[s]->n, probs[s]->C, probs[s]->M, o->precision);
     if (Gs[s] < ns[s]) {
       set_error("track: lanes_per_path=%d is below n=%d", Gs[s], ns[s]);
       return HCX_ERR_INVALID_ARG;
@@ -984,7 +992,7 @@ double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, 
   LN.nseg = 1;
   LN.persistent = 0;
   LN.o = to_opts(&o);
-  const int G = default_lanes(p->n);
+  const int G = default_lanes(p->n, p->C, p->M, HCX_PREC_FP64);
   Plan pl;
   if (choose_kernel(1, &p->n, &G, p->deg, HCX_PREC_FP64, 1, pl) != HCX_OK) return -1.0;
   Seg& S = LN.seg[0];
@@ -1110,15 +1118,16 @@ hcx_status hcx_autotune(hcx_problem* p, const hcx_batch_desc* sample, const hcx_
   std::memset(&r, 0, sizeof(r));
   r.d_roots = reinterpret_cast<hcx_complex*>(roots);
   r.d_flags = flags;
+  const int G0 = default_lanes(p->n, p->C, p->M, o->precision);
   int pow2 = 1;
-  while (pow2 < p->n) pow2 <<= 1;
+  while (pow2 < G0) pow2 <<= 1;
   struct Cand { int G, spec, wpc; };
   std::vector<Cand> cands;
   int occ = 0;
   {
     // occupancy of the default shape, to derive the half-density candidate
     Plan pl;
-    int G = p->n;
+    int G = G0;
     if (choose_kernel(1, &p->n, &G, p->deg, o->precision, 1, pl) == HCX_OK) {
       Launch LN;
       std::memset(&LN, 0, sizeof(LN));
@@ -1129,11 +1138,11 @@ hcx_status hcx_autotune(hcx_problem* p, const hcx_batch_desc* sample, const hcx_
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(pl.fn), 64, pl.lds);
     }
   }
-  for (int spec = 1; spec >= 0; --spec) {
-    cands.push_back({p->n, spec, 0});
-    if (occ > 1) cands.push_back({p->n, spec, std::max(1, occ / 2)});
+  for (int spec = 1; spec >= (G0 == p->n ? 0 : 1); --spec) {
+    cands.push_back({G0, spec, 0});
+    if (occ > 1) cands.push_back({G0, spec, std::max(1, occ / 2)});
   }
-  if (pow2 != p->n && pow2 <= 64) {
+  if (pow2 != G0 && pow2 <= 64) {
     cands.push_back({pow2, 0, 0});
     if (occ > 1) cands.push_back({pow2, 0, std::max(1, occ / 2)});
   }

@@ -486,3 +486,36 @@ def flops_per_path(info, steps, newton, refine):
     refine = np.asarray(refine, dtype=np.float64)
     return (4.0 * steps * info["flops_stage"] + newton * info["flops_newton"]
             + refine * info["flops_newton"] + (refine > 0) * info["flops_coef"])
+
+
+# ---------------------------------------------------------------- reference problem folders
+def ref_read_settings(problem_dir):
+    """gpuhc_settings.yaml of a reference problem folder as a dict."""
+    st = RefSettings()
+    _check(lib().hcx_ref_read_settings(problem_dir.encode(), C.byref(st)), "hcx_ref_read_settings")
+    out = {f: getattr(st, f) for f, _ in RefSettings._fields_}
+    for key in ("problem_name", "gpuhc_type"):
+        out[key] = out[key].decode()
+    return out
+
+
+def ref_read_params(path, cap):
+    out = np.empty(cap, dtype=np.complex128)
+    cnt = C.c_int()
+    _check(lib().hcx_ref_read_params(path.encode(), _ptr(out), cap, C.byref(cnt)), "hcx_ref_read_params")
+    return out[:cnt.value].copy()
+
+
+def ref_read_start_sols(path, n, cap_tracks):
+    out = np.empty((cap_tracks, n), dtype=np.complex128)
+    cnt = C.c_int()
+    _check(lib().hcx_ref_read_start_sols(path.encode(), n, _ptr(out), cap_tracks, C.byref(cnt)),
+           "hcx_ref_read_start_sols")
+    return out[:cnt.value].copy()
+
+
+def ref_read_indices(path, cap):
+    out = np.empty(cap, dtype=np.int32)
+    cnt = C.c_int()
+    _check(lib().hcx_ref_read_indices(path.encode(), _ptr(out), cap, C.byref(cnt)), "hcx_ref_read_indices")
+    return out[:cnt.value].copy()

@@ -376,8 +376,8 @@ hcx_status hcx_random_cn(uint64_t seed, int64_t count, hcx_complex* out);
 hcx_complex hcx_random_gamma(uint64_t seed);
 
 /* ------------------------------------------------------------------------ */
-/* Reference on-disk problem folders (GPU-HC/problems/<p>/, Appendix of the  */
-/* reference format): gpuhc_settings.yaml, start_params.txt,                 */
+/* Reference on-disk problem folders (GPU-HC/problems/<p>/):              */
+/* gpuhc_settings.yaml, start_params.txt,                                    */
 /* target_params.txt, start_sols.txt, dHdx_indx.txt, dHdt_indx.txt.          */
 /* Replaces Data_Reader::Read_* (GPU-HC/magmaHC/Data_Reader.cpp:91-225) and   */
 /* the YAML parsing of GPU_HC_Solver.cpp:44-68.                              */

@@ -50,6 +50,7 @@ def lib():
         _lib.orc_lu_solve.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.orc_p2c_affine.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                         C.c_void_p]
+        _lib.orc_p2c_poly.argtypes = [C.c_int] * 5 + [C.c_void_p] * 4 + [C.c_int, C.c_void_p, C.c_void_p]
         _lib.orc_track.argtypes = [C.POINTER(Tables), C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                    C.POINTER(Opts), C.c_int] + [C.c_void_p] * 8
         _lib.orc_collect.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
@@ -128,6 +129,24 @@ def p2c_affine(affine, p0, p1):
     hx = np.empty((I, Cn + 1, 2), dtype=np.complex128)
     ht = np.empty((I, Cn + 1, 1), dtype=np.complex128)
     lib().orc_p2c_affine(Cn, k1 - 1, _p(affine), _p(p0), _p(p1), I, _p(hx), _p(ht))
+    return hx, ht
+
+
+def p2c_poly(scalar, params, p0, p1, M):
+    """Coefficient-polynomial tables of a parameter-polynomial coefficient map
+    for each target row of p1: hx [I][C+1][M+1], ht [I][C+1][M]."""
+    scalar = np.ascontiguousarray(scalar, dtype=np.complex128)
+    params = np.ascontiguousarray(params, dtype=np.int32)
+    Cn, T = scalar.shape
+    D = params.shape[2]
+    p0 = np.ascontiguousarray(p0, dtype=np.complex128)
+    k = p0.shape[0]
+    p1 = np.ascontiguousarray(p1, dtype=np.complex128).reshape(-1, k)
+    I = p1.shape[0]
+    hx = np.empty((I, Cn + 1, M + 1), dtype=np.complex128)
+    ht = np.empty((I, Cn + 1, M), dtype=np.complex128)
+    if lib().orc_p2c_poly(Cn, T, D, k, M, _p(scalar), _p(params), _p(p0), _p(p1), I, _p(hx), _p(ht)) != 0:
+        raise ValueError("p2c_poly: parameter index out of range or degree above M")
     return hx, ht
 
 

@@ -337,6 +337,50 @@ int orc_p2c_affine(int C, int k, const double* A, const double* p0, const double
   return 0;
 }
 
+// Parameter-polynomial coefficient map c_j(p) = sum_u s[j][u] prod_d p[idx[j][u][d]]
+// (p[k] = 1) on the path p(t) = p0 + t (p1 - p0): the t^m coefficients of c_j
+// (hx [I][C+1][M+1]) and of dc_j/dt (ht [I][C+1][M]); entry C is the constant
+// 1.  The restatement of the generated P2C code (Data_Reader.cpp:59-89 with
+// the createUnivPoly4Coeffs.m layout): each term's product of linear factors
+// is expanded factor by factor.
+int orc_p2c_poly(int C, int T, int D, int k, int M, const double* scal, const int* idx, const double* p0,
+                 const double* p1, int I, double* hx, double* ht) {
+  const cd* sc = reinterpret_cast<const cd*>(scal);
+  const cd* q0 = reinterpret_cast<const cd*>(p0);
+  const cd* q1 = reinterpret_cast<const cd*>(p1);
+  cd* ox = reinterpret_cast<cd*>(hx);
+  cd* ot = reinterpret_cast<cd*>(ht);
+  std::vector<cd> acc(M + 1), term(M + 2);
+  for (int i = 0; i < I; ++i) {
+    for (int j = 0; j <= C; ++j) {
+      std::fill(acc.begin(), acc.end(), cd(0, 0));
+      if (j == C) {
+        acc[0] = cd(1, 0);
+      } else {
+        for (int u = 0; u < T; ++u) {
+          const cd s = sc[(size_t)j * T + u];
+          if (s == cd(0, 0)) continue;
+          std::fill(term.begin(), term.end(), cd(0, 0));
+          term[0] = s;
+          int deg = 0;
+          for (int d = 0; d < D; ++d) {
+            const int a = idx[((size_t)j * T + u) * D + d];
+            if (a == k) continue;
+            if (a < 0 || a > k || deg + 1 > M) return -1;
+            const cd qa = q0[a], da = q1[(size_t)i * k + a] - q0[a];
+            for (int m = deg + 1; m >= 0; --m) term[m] = term[m] * qa + (m > 0 ? term[m - 1] * da : cd(0, 0));
+            ++deg;
+          }
+          for (int m = 0; m <= M; ++m) acc[m] += term[m];
+        }
+      }
+      for (int m = 0; m <= M; ++m) ox[((size_t)i * (C + 1) + j) * (M + 1) + m] = acc[m];
+      for (int m = 0; m < M; ++m) ot[((size_t)i * (C + 1) + j) * M + m] = (double)(m + 1) * acc[m + 1];
+    }
+  }
+  return 0;
+}
+
 // Tracks every (instance, root) path: path = inst * K + root.  hx/ht are
 // [I][(C+1)(M+1)] / [I][(C+1)M].  Outputs per path (any may be NULL except
 // roots and flags).  threads <= 0: hardware concurrency.
