@@ -163,6 +163,117 @@ hcx_status hcx_ref_read_indices(const char* path, int* out, int cap, int* count)
   return HCX_OK;
 }
 
+hcx_status hcx_ref_read_cmap(const char* path, int* C, int* T, int* D, int* k, hcx_complex* scalar, int* params,
+                             int cap_terms) {
+  clear_error();
+  if (!path || !C || !T || !D || !k || (scalar && !params)) {
+    set_error("hcx_ref_read_cmap: invalid arguments");
+    return HCX_ERR_INVALID_ARG;
+  }
+  std::ifstream f(path);
+  if (!f) {
+    set_error("hcx_ref_read_cmap: cannot open %s", path);
+    return HCX_ERR_IO;
+  }
+  int c = 0, t = 0, d = 0, kk = 0;
+  if (!(f >> c >> t >> d >> kk) || c < 1 || t < 1 || d < 1 || kk < 1) {
+    set_error("hcx_ref_read_cmap: %s: bad header (need \"C T D k\")", path);
+    return HCX_ERR_IO;
+  }
+  *C = c;
+  *T = t;
+  *D = d;
+  *k = kk;
+  if (!scalar) return HCX_OK;
+  const long long terms = (long long)c * t;
+  if (terms > cap_terms) {
+    set_error("hcx_ref_read_cmap: %lld terms exceed the buffer (%d)", terms, cap_terms);
+    return HCX_ERR_INVALID_ARG;
+  }
+  for (long long u = 0; u < terms; ++u) {
+    double re, im;
+    if (!(f >> re >> im)) {
+      set_error("hcx_ref_read_cmap: %s: term %lld missing", path, u);
+      return HCX_ERR_IO;
+    }
+    scalar[u] = hcx_complex{re, im};
+    for (int q = 0; q < d; ++q) {
+      int a;
+      if (!(f >> a) || a < 0 || a > kk) {
+        set_error("hcx_ref_read_cmap: %s: term %lld has a bad parameter index", path, u);
+        return HCX_ERR_IO;
+      }
+      params[u * d + q] = a;
+    }
+  }
+  return HCX_OK;
+}
+
+hcx_status hcx_ref_ph_to_p2c(int n, int P, int ht_terms, int ht_parts, const int* ph_ht, int hx_terms, int hx_parts,
+                             const int* ph_hx, int* C, int* ht, int* hx, int* pairs, int cap_pairs) {
+  clear_error();
+  if (n < 1 || P < 1 || ht_terms < 1 || hx_terms < 1 || ht_parts < 4 || hx_parts < 4 || !ph_ht || !ph_hx || !C ||
+      !ht || !hx || (cap_pairs > 0 && !pairs)) {
+    set_error("hcx_ref_ph_to_p2c: invalid arguments");
+    return HCX_ERR_INVALID_ARG;
+  }
+  std::vector<std::pair<int, int>> found;
+  auto pid = [&](int a, int b) -> int {
+    if (a > b) std::swap(a, b);
+    if (a == P && b == P) return -1;
+    for (size_t q = 0; q < found.size(); ++q)
+      if (found[q].first == a && found[q].second == b) return (int)q;
+    found.emplace_back(a, b);
+    return (int)found.size() - 1;
+  };
+  // first pass assigns the numbering; the tables are written with -1 for the
+  // constant and fixed up once C is known
+  const int htp = ht_parts - 1, hxp = hx_parts - 1;
+  for (int i = 0; i < n; ++i)
+    for (int u = 0; u < ht_terms; ++u) {
+      int* o = ht + ((size_t)i * ht_terms + u) * htp;
+      const int s = ph_ht[((size_t)u * ht_parts + 0) * n + i];
+      const int a = ph_ht[((size_t)u * ht_parts + 1) * n + i], b = ph_ht[((size_t)u * ht_parts + 2) * n + i];
+      if (a < 0 || a > P || b < 0 || b > P) {
+        set_error("hcx_ref_ph_to_p2c: dH/dt parameter index out of range");
+        return HCX_ERR_INVALID_ARG;
+      }
+      o[0] = s;
+      o[1] = s != 0 ? pid(a, b) : -1;
+      for (int q = 3; q < ht_parts; ++q) o[q - 1] = ph_ht[((size_t)u * ht_parts + q) * n + i];
+    }
+  for (int j = 0; j < n; ++j)
+    for (int u = 0; u < hx_terms; ++u)
+      for (int i = 0; i < n; ++i) {
+        int* o = hx + (((size_t)i * n + j) * hx_terms + u) * hxp;
+        const size_t base = (((size_t)j * hx_terms + u) * hx_parts) * n + i;
+        const int s = ph_hx[base];
+        const int a = ph_hx[base + (size_t)1 * n], b = ph_hx[base + (size_t)2 * n];
+        if (a < 0 || a > P || b < 0 || b > P) {
+          set_error("hcx_ref_ph_to_p2c: dH/dx parameter index out of range");
+          return HCX_ERR_INVALID_ARG;
+        }
+        o[0] = s;
+        o[1] = s != 0 ? pid(a, b) : -1;
+        for (int q = 3; q < hx_parts; ++q) o[q - 1] = ph_hx[base + (size_t)q * n];
+      }
+  const int c = (int)found.size();
+  if (c > cap_pairs) {
+    set_error("hcx_ref_ph_to_p2c: %d parameter products exceed the pair buffer (%d)", c, cap_pairs);
+    return HCX_ERR_INVALID_ARG;
+  }
+  for (size_t q = 0; q < (size_t)n * ht_terms; ++q)
+    if (ht[q * htp + 1] < 0) ht[q * htp + 1] = c;
+  for (size_t q = 0; q < (size_t)n * n * hx_terms; ++q)
+    if (hx[q * hxp + 1] < 0) hx[q * hxp + 1] = c;
+  for (int q = 0; q < c; ++q) {
+    pairs[2 * q] = found[q].first;
+    pairs[2 * q + 1] = found[q].second;
+  }
+  *C = c;
+  return HCX_OK;
+}
+
 hcx_status hcx_ref_write_converged(const char* path, int n, int num_instances, int num_tracks,
                                    const hcx_complex* h_roots, const uint32_t* h_flags, uint32_t mask) {
   clear_error();

@@ -519,3 +519,31 @@ def ref_read_indices(path, cap):
     cnt = C.c_int()
     _check(lib().hcx_ref_read_indices(path.encode(), _ptr(out), cap, C.byref(cnt)), "hcx_ref_read_indices")
     return out[:cnt.value].copy()
+
+
+def ref_read_cmap(path):
+    """(scalar [C][T], params [C][T][D], k) of a coefficient map file."""
+    Cn, T, D, k = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    _check(lib().hcx_ref_read_cmap(path.encode(), C.byref(Cn), C.byref(T), C.byref(D), C.byref(k), None, None, 0),
+           "hcx_ref_read_cmap")
+    scal = np.empty((Cn.value, T.value), dtype=np.complex128)
+    prm = np.empty((Cn.value, T.value, D.value), dtype=np.int32)
+    _check(lib().hcx_ref_read_cmap(path.encode(), C.byref(Cn), C.byref(T), C.byref(D), C.byref(k), _ptr(scal),
+                                   _ptr(prm), scal.size), "hcx_ref_read_cmap")
+    return scal, prm, k.value
+
+
+def ref_ph_to_p2c(n, P, ph_ht, ph_hx):
+    """C-ABI PH -> P2C conversion: (C, ht [n][htT][htP-1], hx [n][n][hxT][hxP-1], pairs [C][2])."""
+    ph_ht = np.ascontiguousarray(ph_ht, dtype=np.int32)
+    ph_hx = np.ascontiguousarray(ph_hx, dtype=np.int32)
+    htT, htP = ph_ht.shape[0], ph_ht.shape[1]
+    hxT, hxP = ph_hx.shape[1], ph_hx.shape[2]
+    ht = np.empty((n, htT, htP - 1), dtype=np.int32)
+    hx = np.empty((n, n, hxT, hxP - 1), dtype=np.int32)
+    cap = (P + 1) * (P + 2) // 2
+    pairs = np.empty((cap, 2), dtype=np.int32)
+    Cn = C.c_int()
+    _check(lib().hcx_ref_ph_to_p2c(n, P, htT, htP, _ptr(ph_ht), hxT, hxP, _ptr(ph_hx), C.byref(Cn), _ptr(ht),
+                                   _ptr(hx), _ptr(pairs), cap), "hcx_ref_ph_to_p2c")
+    return Cn.value, ht, hx, pairs[:Cn.value].copy()

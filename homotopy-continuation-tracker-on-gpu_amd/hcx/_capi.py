@@ -167,6 +167,9 @@ SIGNATURES = {
     "hcx_ref_read_params": (C.c_int, [C.c_char_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "hcx_ref_read_start_sols": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "hcx_ref_read_indices": (C.c_int, [C.c_char_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "hcx_ref_read_cmap": (C.c_int, [C.c_char_p] + [C.POINTER(C.c_int)] * 4 + [C.c_void_p, C.c_void_p, C.c_int]),
+    "hcx_ref_ph_to_p2c": (C.c_int, [C.c_int] * 4 + [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int),
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "hcx_ref_write_converged": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                           C.c_uint32]),
 }

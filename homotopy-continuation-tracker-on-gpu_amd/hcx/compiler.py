@@ -34,7 +34,8 @@ from fractions import Fraction
 import numpy as np
 
 __all__ = ["SystemDef", "parse_system", "compile_system", "infer_coefficient_map", "coefficient_map_arrays",
-           "coefficient_tables", "ph_to_p2c", "read_table", "read_settings", "read_complex", "load_problem_dir"]
+           "coefficient_tables", "ph_to_p2c", "read_table", "read_settings", "read_complex", "load_problem_dir",
+           "write_coefficient_map", "compile_problem_dir"]
 
 
 class SystemDef:
@@ -440,3 +441,61 @@ def load_problem_dir(path):
     out["target_params"] = read_complex(os.path.join(path, "target_params.txt"))
     out["start_sols"] = read_complex(os.path.join(path, "start_sols.txt")).reshape(-1, n)
     return out
+
+
+def write_coefficient_map(path, scalar, params, k):
+    """Coefficient map file read by hcx_ref_read_cmap: "C T D k", then per
+    coefficient j and term u one line "re im i_1 .. i_D"."""
+    scalar = np.asarray(scalar, dtype=np.complex128)
+    params = np.asarray(params)
+    Cn, T = scalar.shape
+    D = params.shape[2]
+    with open(path, "w") as fh:
+        fh.write(f"{Cn} {T} {D} {k}\n")
+        for j in range(Cn):
+            for u in range(T):
+                z = scalar[j, u]
+                idx = " ".join(str(int(a)) for a in params[j, u])
+                fh.write(f"{float(z.real)!r} {float(z.imag)!r} {idx}\n")
+
+
+def compile_problem_dir(problem_dir, sys_file, out=None):
+    """Coefficient map of a P2C problem folder in its own numbering, from the
+    system definition; writes it to `out` (default <problem_dir>/coeff_map.txt
+    when writable) and returns (scalar, params, k)."""
+    import os
+    with open(sys_file) as fh:
+        sd = parse_system(fh.read(), os.path.basename(problem_dir.rstrip("/")))
+    d = load_problem_dir(problem_dir)
+    st = d["settings"]
+    n = st["Num_Of_Vars"]
+    if st.get("GPUHC_Type") != "P2C":
+        raise ValueError("only P2C problems carry a coefficient map (PH problems convert with ph_to_p2c)")
+    C = st["Num_Of_Coeffs_From_Params"]
+    ht = d["ht"].reshape(n, st["dHdt_Max_Terms"], st["dHdt_Max_Parts"])
+    hx = d["hx"].reshape(n, n, st["dHdx_Max_Terms"], st["dHdx_Max_Parts"])
+    coef = infer_coefficient_map(sd, ht, hx, C)
+    k = st["Num_Of_Params"]
+    scal, prm = coefficient_map_arrays(coef, sd.ps[:k])
+    if out is None:
+        out = os.path.join(problem_dir, "coeff_map.txt")
+    write_coefficient_map(out, scal, prm, k)
+    return scal, prm, k
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(
+        prog="python -m hcx.compiler",
+        description="Coefficient map of a reference P2C problem folder from its system definition")
+    ap.add_argument("problem_dir", help="problems/<p>/ folder (gpuhc_settings.yaml, dHdt_indx.txt, ...)")
+    ap.add_argument("sys_file", help="sys_<p>.m polynomial system definition")
+    ap.add_argument("-o", "--out", help="output file (default <problem_dir>/coeff_map.txt)")
+    a = ap.parse_args(argv)
+    scal, prm, k = compile_problem_dir(a.problem_dir, a.sys_file, a.out)
+    print(f"coefficient map: {scal.shape[0]} coefficients, {scal.shape[1]} terms x degree {prm.shape[2]}, k={k}")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

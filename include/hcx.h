@@ -400,6 +400,31 @@ hcx_status hcx_ref_read_indices(const char* path, int* out, int cap, int* count)
 hcx_status hcx_ref_write_converged(const char* path, int n, int num_instances, int num_tracks,
                                    const hcx_complex* h_roots, const uint32_t* h_flags, uint32_t mask);
 
+/* Parameter-polynomial coefficient map file (written by the problem compiler,
+ * hcx/compiler.py; it takes the place of the generated p2c-<p>.h of a P2C
+ * problem, GPU-HC/magmaHC/PHC_Coeffs/, called from Data_Reader.cpp:59-89):
+ *   line 1: "C T D k"; then C*T lines "re im i_1 .. i_D", coefficient-major,
+ * meaning c_j(p) = sum_u scalar[j][u] prod_d p[i_d] with p[k] = 1.
+ * With scalar == NULL only the sizes are returned; otherwise scalar holds
+ * cap_terms >= C*T entries and params cap_terms*D. */
+hcx_status hcx_ref_read_cmap(const char* path, int* C, int* T, int* D, int* k, hcx_complex* scalar, int* params,
+                             int cap_terms);
+
+/* PH-type problem (GPUHC_Type: PH, trifocal_2op1p_30x30) -> P2C form.
+ * PH tables are transposed, equation index fastest:
+ *   ph_ht [ht_terms][ht_parts][n], ph_hx [n var][hx_terms][hx_parts][n eq],
+ * parts = {scalar, p_a, p_b, x ...}, parameter index P = the constant 1
+ * (dev-eval-indxing-trifocal_2op1p_30x30.cuh:44-168,
+ *  kernel_HC_Solver_trifocal_2op1p_30x30.cu:124-139).  Each distinct unordered
+ * pair {a, b} other than {P, P} becomes one coefficient p_a * p_b, numbered in
+ * order of first use (dH/dt table first, then dH/dx); {P, P} and padding take
+ * the constant index C.  Outputs: *C, ht [n][ht_terms][ht_parts-1],
+ * hx [n][n][hx_terms][hx_parts-1] in the P2C layouts, pairs [C][2]
+ * (cap_pairs >= C).  The coefficient map is then scalar 1, params pairs[j]
+ * with k = P (degree 2, max_order_t = 2). */
+hcx_status hcx_ref_ph_to_p2c(int n, int P, int ht_terms, int ht_parts, const int* ph_ht, int hx_terms, int hx_parts,
+                             const int* ph_hx, int* C, int* ht, int* hx, int* pairs, int cap_pairs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,121 @@
+"""C++ host lifecycle (hcx::GPU_HC_Solver) and its CLI, hcx_solve, on problem
+folders in the reference's on-disk format written from the golden fixtures;
+the coefficient-map file and PH conversion paths of the C-ABI."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import hcx
+import oracle
+from hcx import compiler as cc
+from test_reference_problems import load
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "homotopy-continuation-tracker-on-gpu_amd", "hcx_solve")
+
+
+def write_problem_dir(d, path, with_cmap=True):
+    """A problems/<p>/ folder in the reference's formats (tab separated)."""
+    os.makedirs(path, exist_ok=True)
+    n = d["n"]
+    st = ["%YAML:1.0", f"problem_name: {d['name']}", f"GPUHC_Type: {d['kind']}",
+          f"GPUHC_Max_Steps: {d['max_steps']}", f"GPUHC_Max_Correction_Steps: {d['max_corr']}",
+          f"GPUHC_Num_Of_Steps_to_Increase_Delta_t: {d['incr_steps']}", f"Num_Of_Vars: {n}",
+          f"Num_Of_Params: {d['k']}", f"Num_Of_Tracks: {len(d['start_sols'])}",
+          f"dHdx_Max_Terms: {int(d['hx_terms'])}", f"dHdx_Max_Parts: {int(d['hx_parts'])}",
+          f"dHdt_Max_Terms: {int(d['ht_terms'])}", f"dHdt_Max_Parts: {int(d['ht_parts'])}",
+          f"Max_Order_Of_T: {d['M']}", "RANSAC_Dataset: None"]
+    if d["kind"] == "P2C":
+        st.append(f"Num_Of_Coeffs_From_Params: {d['C']}")
+    with open(os.path.join(path, "gpuhc_settings.yaml"), "w") as fh:
+        fh.write("\n".join(st) + "\n")
+
+    def cplx(name, z):
+        with open(os.path.join(path, name), "w") as fh:
+            fh.write("\n".join(f"{float(v.real)!r}\t{float(v.imag)!r}" for v in np.ravel(z)))
+
+    cplx("start_params.txt", d["start_params"])
+    cplx("target_params.txt", d["target_params"])
+    cplx("start_sols.txt", d["start_sols"])
+    for name, tab, width in (("dHdt_indx.txt", d["ht"], int(d["ht_parts"]) * int(d["ht_terms"])),
+                             ("dHdx_indx.txt", d["hx"], int(d["hx_parts"]) * int(d["hx_terms"]) * n)):
+        rows = np.asarray(tab).reshape(-1, width if d["kind"] == "P2C" else n)
+        with open(os.path.join(path, name), "w") as fh:
+            fh.write("\n".join("\t".join(str(int(v)) for v in r) + "\t" for r in rows))
+    if with_cmap and d["kind"] == "P2C":
+        cc.write_coefficient_map(os.path.join(path, "coeff_map.txt"), *d["cmap"], d["k"])
+
+
+def test_cmap_file_round_trip(tmp_path):
+    d = load("dual_reciever_TDOA_5pt")
+    f = str(tmp_path / "coeff_map.txt")
+    cc.write_coefficient_map(f, *d["cmap"], d["k"])
+    scal, prm, k = hcx.ref_read_cmap(f)
+    assert k == d["k"]
+    assert np.array_equal(scal, d["cmap"][0]) and np.array_equal(prm, d["cmap"][1])
+
+
+def test_ph_conversion_c_abi_equals_python():
+    d = load("trifocal_2op1p_30x30")
+    n, P = d["n"], d["k"]
+    ph_ht = d["ht"].reshape(int(d["ht_terms"]), int(d["ht_parts"]), n)
+    ph_hx = d["hx"].reshape(n, int(d["hx_terms"]), int(d["hx_parts"]), n)
+    Cn, ht, hx, pairs = hcx.ref_ph_to_p2c(n, P, ph_ht, ph_hx)
+    py = cc.ph_to_p2c(n, P, ph_ht, ph_hx)
+    assert Cn == py["C"]
+    assert np.array_equal(ht, py["ht"]) and np.array_equal(hx, py["hx"])
+    assert np.array_equal(pairs, py["cmap"][1][:, 0, :])
+
+
+def test_cli_help():
+    out = subprocess.run([CLI, "--help"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and "hcx_solve -p" in out.stdout
+
+
+def test_cli_reports_missing_cmap(tmp_path):
+    d = load("P3P")
+    write_problem_dir(d, str(tmp_path / "P3P"), with_cmap=False)
+    out = subprocess.run([CLI, "-p", str(tmp_path / "P3P")], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 1
+    assert "coefficient map" in out.stderr and "hcx.compiler" in out.stderr
+
+
+def parse_summary(text):
+    vals = {}
+    for line in text.splitlines():
+        for key in ("converged (t = 1)", "refined in FP64", "infinity failures", "real solutions",
+                    "unique solutions"):
+            if line.startswith("- " + key):
+                vals[key] = int(line.split(":")[-1])
+    return vals
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["P3P", "generalized_3views_4pts", "trifocal_2op1p_30x30"])
+def test_cli_solves_reference_folder(tmp_path, name):
+    """hcx_solve on a reference-format folder: counts equal the oracle's on
+    the same paths (FP64 and mixed), and the written solution file holds the
+    refined roots."""
+    d = load(name)
+    folder = str(tmp_path / name)
+    write_problem_dir(d, folder)
+    op = oracle.OracleProblem(d["n"], d["C"], d["hx4"], d["ht3"], d["M"])
+    ohx, oht = oracle.p2c_poly(*d["cmap"], d["start_params"], d["target_params"], d["M"])
+    o = op.track(d["start_sols"], ohx, oht, threads=16, max_steps=d["max_steps"], max_corr=d["max_corr"],
+                 incr_steps=d["incr_steps"])
+    counts, _ = oracle.collect(d["n"], 1, len(d["start_sols"]), o["roots"], o["flags"])
+    unstable_ok = name == "generalized_3views_4pts"  # 100 corrector iterations per step (see the GPU parity tests)
+    for extra in ([], ["--fp64"]):
+        print(f"hcx_solve {name} {extra}", flush=True)
+        out = subprocess.run([CLI, "-p", folder, "-o", str(tmp_path)] + extra, capture_output=True, text=True,
+                             timeout=300)
+        assert out.returncode == 0, out.stderr
+        v = parse_summary(out.stdout)
+        slack = 12 if unstable_ok else 0
+        assert abs(v["refined in FP64"] - counts["refined"]) <= slack, (v, counts)
+        assert abs(v["converged (t = 1)"] - counts["tracked"]) <= slack
+        assert abs(v["real solutions"] - counts["real"]) <= slack
+        text = open(str(tmp_path / "GPU_Converged_HC_tracks.txt")).read()
+        assert text.count("\n\n") >= v["refined in FP64"] - 1
