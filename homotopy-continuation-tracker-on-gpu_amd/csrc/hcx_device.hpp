@@ -159,7 +159,8 @@ __device__ __forceinline__ T group_sum(T v, const Group& g, int lane) {
 
 // ------------------------------------------------------------ device tables
 // One table term: scalar * c[coef] * prod x[var_q]; var index n reads the
-// constant 1 stored at x[n] (reference table convention, Appendix A format).
+// constant 1 stored at x[n] (reference table convention,
+// dev-eval-indxing-generalized_3views_4pts.cuh:70-120).
 struct __align__(16) Term {
   int s;
   int c;
