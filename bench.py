@@ -467,31 +467,16 @@ def bench_config4(args):
     print(json.dumps(out))
 
 
-def bench_config5(args):
-    """Serving loop at a fixed Poisson request rate (2000 req/s by default):
-    six resident quadratic types n in {3,6,9,12,16,30}, one parameter instance
-    per request, continuous batching with admission by HBM budget."""
+def _serve(srv, tids, types, rate, duration, seed):
+    """Poisson arrivals at `rate` for `duration` seconds; returns the stats."""
     import numpy as np
-    import hcx
     from hcx import workloads
-    rate = float(os.environ.get("HCX_SERVE_RATE", "2000"))
-    duration = float(os.environ.get("HCX_SERVE_SECONDS", "30"))
-    log("config5: seeding start roots of six types")
-    types = workloads.quadratic_types(workloads.CONFIG5_TYPES, seed=args.seed)
-    probs = [hcx.Problem.from_system(t["system"], t["start"], p0=t["p0"]) for t in types]
-    opts = hcx.default_opts(precision=hcx.PREC_MIXED)
-    srv = hcx.Server(opts, hbm_budget_bytes=4 << 30, max_batch_requests=64, streams=4)
-    tids = [srv.add_type(p) for p in probs]
-    rng = np.random.default_rng(args.seed)
-    # warm-up: every type once
-    for ti, ty in enumerate(types):
-        srv.submit(tids[ti], workloads.targets(ty["k"], 1, 90000 + ti)[0])
-    srv.drain()
+    rng = np.random.default_rng(seed)
     srv.reset_stats()
     n_req = int(rate * duration)
     gaps = rng.exponential(1.0 / rate, n_req)
     kinds = rng.integers(0, len(types), n_req)
-    params = [workloads.targets(types[k]["k"], 1, 100000 + i)[0] for i, k in enumerate(kinds)]
+    params = [workloads.targets(types[k]["k"], 1, seed * 1000003 + i)[0] for i, k in enumerate(kinds)]
     pending = []
     t_start = time.perf_counter()
     t_next = t_start
@@ -519,20 +504,81 @@ def bench_config5(args):
             log(f"config5: t={last_log - t_start:.1f}s submitted {st['submitted']} completed {st['completed']} "
                 f"inflight {st['inflight_requests']} p99 {st['p99_ms']:.2f} ms")
     t_sub = time.perf_counter() - t_start
-    srv.drain(300.0)
+    srv.drain(600.0)
     t_end = time.perf_counter() - t_start
     for rid, k in pending:
         srv.result(rid, k)
     st = srv.stats()
-    out = {"metric": METRIC, "value": st["converged_paths"] / t_end, "unit": "converged paths/s",
-           "config": {"workload": f"config 5: serving loop, Poisson arrivals at {rate:.0f} req/s for {duration:.0f} s, "
-                                  f"types (n,K)={workloads.CONFIG5_TYPES} drawn uniformly, mixed FP32/FP64",
-                      "hbm_budget_bytes": 4 << 30, "streams": 4, "max_batch_requests": 64},
-           "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "max_ms": st["max_ms"],
-           "requests_per_s": st["completed"] / t_end, "offered_requests_per_s": rate,
-           "submit_seconds": t_sub, "total_seconds": t_end, "max_submit_lateness_ms": late * 1e3,
-           "stats": st}
+    return {"offered_requests_per_s": rate, "seconds": duration, "requests": n_req,
+            "requests_per_s": st["completed"] / t_end, "converged_paths_per_s": st["converged_paths"] / t_end,
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "max_ms": st["max_ms"],
+            "submit_seconds": t_sub, "total_seconds": t_end, "max_submit_lateness_ms": late * 1e3, "stats": st}
+
+
+def _fp64_per_problem_mix(probs, types, per_type, seed):
+    """The plain FP64 per-problem launch (one static launch per request, the
+    reference's launch shape) over the same type mix: converged paths/s with
+    equally many requests of every type."""
+    import torch
+    import hcx
+    from hcx import workloads
+    o = hcx.default_opts(precision=hcx.PREC_FP64)
+    conv = 0
+    secs = 0.0
+    per = []
+    for ti, (p, ty) in enumerate(zip(probs, types)):
+        res = hcx.Result(p.num_tracks, p.n, 1)
+        tg = workloads.targets(ty["k"], per_type + 1, seed + 7 * ti)
+        b0 = hcx.Batch(p, 1, target_params=tg[:1])
+        hcx.track(p, b0, o, result=res, static=True)
+        torch.cuda.synchronize()
+        batches = [hcx.Batch(p, 1, target_params=tg[i + 1:i + 2]) for i in range(per_type)]
+        c = 0
+        t0 = time.perf_counter()
+        for b in batches:
+            hcx.track(p, b, o, result=res, static=True)
+            c += hcx.collect(res, o)["refined"]
+        el = time.perf_counter() - t0
+        conv += c
+        secs += el
+        per.append({"n": ty["n"], "K": ty["K"], "ms_per_request": el / per_type * 1e3})
+    return {"converged_paths_per_s": conv / secs, "requests_per_type": per_type, "per_type": per}
+
+
+def bench_config5(args):
+    """Serving loop: Poisson arrivals of single-instance requests over six
+    resident quadratic types n in {3,6,9,12,16,30}, continuous batching with
+    admission by HBM budget.  Runs at the offered 2000 req/s (throughput at
+    saturation) and at a fixed sustainable rate (latency), and compares the
+    served converged paths/s with the plain FP64 per-problem launch over the
+    same mix."""
+    import hcx
+    from hcx import workloads
+    rate = float(os.environ.get("HCX_SERVE_RATE", "2000"))
+    duration = float(os.environ.get("HCX_SERVE_SECONDS", "20"))
+    fixed = float(os.environ.get("HCX_SERVE_FIXED_RATE", "400"))
+    log("config5: seeding start roots of six types")
+    types = workloads.quadratic_types(workloads.CONFIG5_TYPES, seed=args.seed)
+    probs = [hcx.Problem.from_system(t["system"], t["start"], p0=t["p0"]) for t in types]
+    opts = hcx.default_opts(precision=hcx.PREC_MIXED)
+    srv = hcx.Server(opts, hbm_budget_bytes=4 << 30, max_batch_requests=64, streams=4)
+    tids = [srv.add_type(p) for p in probs]
+    for ti, ty in enumerate(types):
+        srv.submit(tids[ti], workloads.targets(ty["k"], 1, 90000 + ti)[0])
+    srv.drain()
+    saturated = _serve(srv, tids, types, rate, duration, seed=args.seed)
+    steady = _serve(srv, tids, types, fixed, duration, seed=args.seed + 1)
     srv.close()
+    log("config5: plain FP64 per-problem launches over the same mix")
+    fp64 = _fp64_per_problem_mix(probs, types, per_type=24, seed=args.seed * 31)
+    out = {"metric": METRIC, "value": saturated["converged_paths_per_s"], "unit": "converged paths/s",
+           "config": {"workload": f"config 5: serving loop, Poisson arrivals, types (n,K)={workloads.CONFIG5_TYPES} "
+                                  f"drawn uniformly, one instance per request, mixed FP32/FP64",
+                      "hbm_budget_bytes": 4 << 30, "streams": 4, "max_batch_requests": 64},
+           "at_offered_rate": saturated, "at_fixed_rate": steady,
+           "p50_ms": steady["p50_ms"], "p99_ms": steady["p99_ms"], "latency_rate_req_s": fixed,
+           "fp64_per_problem": fp64,
+           "speedup_vs_fp64_per_problem": saturated["converged_paths_per_s"] / fp64["converged_paths_per_s"]}
     print(json.dumps(out))
 
 
