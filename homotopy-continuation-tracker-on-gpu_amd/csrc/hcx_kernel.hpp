@@ -61,6 +61,8 @@ struct Seg {
   const double2* start;        // [K][start_stride]
   const double2* chx;          // [I][C+1][M+1]
   const double2* cht;          // [I][C+1][M]
+  const int* cslot;            // [C+1] LDS slot of coefficient j (the tables hold slots)
+  const int* cinv;             // [C+1] coefficient of LDS slot q
   unsigned int* counter;
   const SegOut* out;           // device copy written by init_kernel
   const int* plist;            // optional: track only these paths (count in *plist_count)
@@ -112,15 +114,18 @@ __device__ __forceinline__ void eval_jac_row(const Seg& S, int gle, const cx<T>*
 
 // Row gl of sum_k s * coef[c] * prod x over the dHdt table: H when coef = c(t),
 // dH/dt when coef = dc/dt (…cuh:89-120).
+// coef is an LDS table indexed by slot, or (remap != nullptr) a global table
+// indexed by coefficient.
 template <int DEG, class T>
-__device__ __forceinline__ cx<T> eval_t_row(const Seg& S, int gle, const cx<T>* sx, const cx<T>* coef) {
+__device__ __forceinline__ cx<T> eval_t_row(const Seg& S, int gle, const cx<T>* sx, const cx<T>* coef,
+                                            const int* remap = nullptr) {
   const int n = S.n;
   cx<T> acc = cmk<T>(0, 0);
   const Term* tp = S.ht + gle;
 #pragma unroll 2
   for (int k = 0; k < S.htT; ++k) {
     const Term tm = tp[(size_t)k * n];
-    cx<T> v = cscale(coef[tm.c], (T)tm.s);
+    cx<T> v = cscale(coef[remap ? remap[tm.c] : tm.c], (T)tm.s);
 #pragma unroll
     for (int q = 0; q < DEG; ++q) v = cmul(v, sx[term_var(tm, q)]);
     acc = cadd(acc, v);
@@ -130,6 +135,10 @@ __device__ __forceinline__ cx<T> eval_t_row(const Seg& S, int gle, const cx<T>* 
 
 // Coefficient polynomials in t of the group's instance into LDS
 // (eval_parameter_homotopy, …cuh:34-67), computed in FP64 and stored as T.
+// Coefficient j goes to slot cslot[j]: slots follow the order in which the
+// rows of a group read them (term by term), so the lanes of a group that read
+// term k of their rows together hit consecutive LDS words instead of words one
+// equation apart (a bank conflict per lane pair when equations have many terms).
 template <class T>
 __device__ __forceinline__ void stage_coeffs(const Seg& S, const Group& g, bool act, int inst, double t,
                                              unsigned char* scb, bool with_dot) {
@@ -142,11 +151,12 @@ __device__ __forceinline__ void stage_coeffs(const Seg& S, const Group& g, bool 
   if (M == 1) {
     // c(t) = c0 + t c1, dc/dt = Ht[j][0]
     for (int j = g.gl; j < C1; j += g.G) {
+      const int q = S.cslot[j];
       const double2 a = hx[2 * (size_t)j], b = hx[2 * (size_t)j + 1];
-      sc[j] = cmk<T>((T)(a.x + t * b.x), (T)(a.y + t * b.y));
+      sc[q] = cmk<T>((T)(a.x + t * b.x), (T)(a.y + t * b.y));
       if (with_dot) {
         const double2 d = ht[j];
-        sd[j] = cmk<T>((T)d.x, (T)d.y);
+        sd[q] = cmk<T>((T)d.x, (T)d.y);
       }
     }
     return;
@@ -160,7 +170,8 @@ __device__ __forceinline__ void stage_coeffs(const Seg& S, const Group& g, bool 
       cr = nr;
       ci = ni;
     }
-    sc[j] = cmk<T>((T)cr, (T)ci);
+    const int q = S.cslot[j];
+    sc[q] = cmk<T>((T)cr, (T)ci);
     if (with_dot) {
       double2 d = ht[(size_t)j * M + M - 1];
       double dr = d.x, di = d.y;
@@ -170,7 +181,7 @@ __device__ __forceinline__ void stage_coeffs(const Seg& S, const Group& g, bool 
         dr = nr;
         di = ni;
       }
-      sd[j] = cmk<T>((T)dr, (T)di);
+      sd[q] = cmk<T>((T)dr, (T)di);
     }
   }
 }
@@ -343,7 +354,7 @@ __device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, cons
     HCX_WAVE_SYNC();
     cx<T> A[NMAX];
     eval_jac_row<NMAX, DEG, T>(S, gle, sx, sc, A);
-    const cx<T> bt = eval_t_row<DEG, T>(S, gle, sx, sd);
+    const cx<T> bt = eval_t_row<DEG, T>(S, gle, sx, sd, sd_ext ? S.cinv : nullptr);
     bool sing;
     const cx<T> k = group_lu_solve<NMAX, T>(A, cmk<T>(-bt.re, -bt.im), g, n, row, L.template piv<T>(), sing);
     kprev = ccast<double>(k);

@@ -15,6 +15,8 @@ struct hcx_problem {
   hcx::dev::Term* d_hx = nullptr;
   hcx::dev::Term* d_ht = nullptr;
   int* d_hx_cnt = nullptr;
+  int* d_cslot = nullptr;              // [C+1] LDS slot of each coefficient
+  int* d_cinv = nullptr;               // [C+1] coefficient of each slot
   double2* d_start = nullptr;
   double2* d_affine = nullptr;
   double2* d_p0 = nullptr;

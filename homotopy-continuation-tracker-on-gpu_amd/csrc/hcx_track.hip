@@ -414,6 +414,8 @@ void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, 
   S.hx = p->d_hx;
   S.ht = p->d_ht;
   S.hx_cnt = p->d_hx_cnt;
+  S.cslot = p->d_cslot;
+  S.cinv = p->d_cinv;
   S.start = p->d_start;
   S.start_stride = p->n;
   S.counter = p->d_counter;
@@ -576,6 +578,32 @@ hcx_status hcx_problem_create(const hcx_problem_desc* d, hcx_problem** out) {
     return HCX_ERR_UNSUPPORTED;
   }
 
+  // LDS slots of the coefficients in the order the lanes of a group read them:
+  // dH/dt terms term by term across the equations, then dH/dx terms, then the rest
+  std::vector<int> cslot(C + 1, -1), cinv(C + 1, -1);
+  int next_slot = 0;
+  auto take = [&](dev::Term& tm) {
+    if (tm.s == 0) return;
+    if (cslot[tm.c] < 0) {
+      cslot[tm.c] = next_slot;
+      cinv[next_slot] = tm.c;
+      ++next_slot;
+    }
+  };
+  for (int tt = 0; tt < d->ht_terms; ++tt)
+    for (int i = 0; i < n; ++i) take(ht[(size_t)tt * n + i]);
+  for (int j = 0; j < n; ++j)
+    for (int tt = 0; tt < d->hx_terms; ++tt)
+      for (int i = 0; i < n; ++i) take(hx[((size_t)j * d->hx_terms + tt) * n + i]);
+  for (int c = 0; c <= C; ++c)
+    if (cslot[c] < 0) {
+      cslot[c] = next_slot;
+      cinv[next_slot] = c;
+      ++next_slot;
+    }
+  for (auto& tm : ht) tm.c = cslot[tm.c];
+  for (auto& tm : hx) tm.c = cslot[tm.c];
+
   auto* p = new hcx_problem();
   (void)hipGetDevice(&p->device);
   p->n = n;
@@ -597,6 +625,12 @@ hcx_status hcx_problem_create(const hcx_problem_desc* d, hcx_problem** out) {
   if ((e = hipMalloc(&p->d_hx, hx.size() * sizeof(dev::Term))) != hipSuccess) return fail(e, "hipMalloc hx");
   if ((e = hipMalloc(&p->d_ht, ht.size() * sizeof(dev::Term))) != hipSuccess) return fail(e, "hipMalloc ht");
   if ((e = hipMalloc(&p->d_hx_cnt, n * sizeof(int))) != hipSuccess) return fail(e, "hipMalloc hx_cnt");
+  if ((e = hipMalloc(&p->d_cslot, (C + 1) * sizeof(int))) != hipSuccess) return fail(e, "hipMalloc cslot");
+  if ((e = hipMalloc(&p->d_cinv, (C + 1) * sizeof(int))) != hipSuccess) return fail(e, "hipMalloc cinv");
+  if ((e = hipMemcpy(p->d_cslot, cslot.data(), (C + 1) * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "copy cslot");
+  if ((e = hipMemcpy(p->d_cinv, cinv.data(), (C + 1) * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "copy cinv");
   if ((e = hipMalloc(&p->d_start, (size_t)p->K * n * sizeof(double2))) != hipSuccess) return fail(e, "hipMalloc start");
   if ((e = hipMalloc(&p->d_counter, 64 * sizeof(unsigned int))) != hipSuccess) return fail(e, "hipMalloc counter");
   if ((e = hipMalloc(&p->d_out, sizeof(SegOut))) != hipSuccess) return fail(e, "hipMalloc out");
@@ -680,6 +714,8 @@ hcx_status hcx_problem_destroy(hcx_problem* p) {
   if (p->d_hx) (void)hipFree(p->d_hx);
   if (p->d_ht) (void)hipFree(p->d_ht);
   if (p->d_hx_cnt) (void)hipFree(p->d_hx_cnt);
+  if (p->d_cslot) (void)hipFree(p->d_cslot);
+  if (p->d_cinv) (void)hipFree(p->d_cinv);
   if (p->d_start) (void)hipFree(p->d_start);
   if (p->d_affine) (void)hipFree(p->d_affine);
   if (p->d_p0) (void)hipFree(p->d_p0);
