@@ -172,5 +172,12 @@ __device__ __forceinline__ int term_var(const Term& t, int q) {
   return q < 4 ? (int)((t.v03 >> (8 * q)) & 0xffu) : (int)((t.v47 >> (8 * (q - 4))) & 0xffu);
 }
 
+// Compact 8-byte form of a term for tables held in LDS: x = coefficient slot
+// (bits 0-15) | scalar (bits 16-31, two's complement); y = variable indices
+// 0..3, 8 bits each (terms of degree <= 4).
+__device__ __forceinline__ int cterm_c(uint2 t) { return (int)(t.x & 0xffffu); }
+__device__ __forceinline__ int cterm_s(uint2 t) { return (int)t.x >> 16; }
+__device__ __forceinline__ int cterm_var(uint2 t, int q) { return (int)((t.y >> (8 * q)) & 0xffu); }
+
 }  // namespace dev
 }  // namespace hcx

@@ -63,6 +63,7 @@ struct Seg {
   const double2* cht;          // [I][C+1][M]
   const int* cslot;            // [C+1] LDS slot of coefficient j (the tables hold slots)
   const int* cinv;             // [C+1] coefficient of LDS slot q
+  const uint2* ctab;           // compact hx then ht tables (LDS-table instances)
   unsigned int* counter;
   const SegOut* out;           // device copy written by init_kernel
   const int* plist;            // optional: track only these paths (count in *plist_count)
@@ -82,6 +83,8 @@ struct Launch {
   int nseg;
   int first_block[kMaxSeg + 1];
   int persistent;
+  int tab_lds;                 // bytes of dynamic LDS before the group regions (compact tables)
+  int dot_lds;                 // FP64 mode, M = 1: stage dc/dt in LDS (else read it from global)
   Opts o;
 };
 
@@ -89,23 +92,36 @@ namespace dev {
 
 // ------------------------------------------------------------ evaluation
 // Row gl of dH/dx (dev-eval-indxing-generalized_3views_4pts.cuh:70-86).
-template <int NMAX, int DEG, class T>
+// LT: the tables are the compact copy in LDS at lt (hx part first).
+template <int NMAX, int DEG, class T, bool LT>
 __device__ __forceinline__ void eval_jac_row(const Seg& S, int gle, const cx<T>* sx, const cx<T>* sc,
-                                             cx<T> (&A)[NMAX]) {
+                                             cx<T> (&A)[NMAX], const uint2* lt) {
   const int n = S.n;
 #pragma unroll
   for (int j = 0; j < NMAX; ++j) {
     cx<T> acc = cmk<T>(0, 0);
     if (j < n) {
       const int cnt = S.hx_cnt[j];
-      const Term* tp = S.hx + (size_t)j * S.hxT * n + gle;
-#pragma unroll 2
-      for (int k = 0; k < cnt; ++k) {
-        const Term tm = tp[(size_t)k * n];
-        cx<T> v = cscale(sc[tm.c], (T)tm.s);
+      if constexpr (LT) {
+        const uint2* tp = lt + (size_t)j * S.hxT * n + gle;
+#pragma unroll 4
+        for (int k = 0; k < cnt; ++k) {
+          const uint2 tm = tp[(size_t)k * n];
+          cx<T> v = cscale(sc[cterm_c(tm)], (T)cterm_s(tm));
 #pragma unroll
-        for (int q = 0; q < DEG - 1; ++q) v = cmul(v, sx[term_var(tm, q)]);
-        acc = cadd(acc, v);
+          for (int q = 0; q < DEG - 1; ++q) v = cmul(v, sx[cterm_var(tm, q)]);
+          acc = cadd(acc, v);
+        }
+      } else {
+        const Term* tp = S.hx + (size_t)j * S.hxT * n + gle;
+#pragma unroll 2
+        for (int k = 0; k < cnt; ++k) {
+          const Term tm = tp[(size_t)k * n];
+          cx<T> v = cscale(sc[tm.c], (T)tm.s);
+#pragma unroll
+          for (int q = 0; q < DEG - 1; ++q) v = cmul(v, sx[term_var(tm, q)]);
+          acc = cadd(acc, v);
+        }
       }
     }
     A[j] = acc;
@@ -116,19 +132,32 @@ __device__ __forceinline__ void eval_jac_row(const Seg& S, int gle, const cx<T>*
 // dH/dt when coef = dc/dt (…cuh:89-120).
 // coef is an LDS table indexed by slot, or (remap != nullptr) a global table
 // indexed by coefficient.
-template <int DEG, class T>
+template <int DEG, class T, bool LT>
 __device__ __forceinline__ cx<T> eval_t_row(const Seg& S, int gle, const cx<T>* sx, const cx<T>* coef,
-                                            const int* remap = nullptr) {
+                                            const uint2* lt, const int* remap = nullptr) {
   const int n = S.n;
   cx<T> acc = cmk<T>(0, 0);
-  const Term* tp = S.ht + gle;
-#pragma unroll 2
-  for (int k = 0; k < S.htT; ++k) {
-    const Term tm = tp[(size_t)k * n];
-    cx<T> v = cscale(coef[remap ? remap[tm.c] : tm.c], (T)tm.s);
+  if constexpr (LT) {
+    const uint2* tp = lt + (size_t)n * S.hxT * n + gle;
+#pragma unroll 4
+    for (int k = 0; k < S.htT; ++k) {
+      const uint2 tm = tp[(size_t)k * n];
+      const int c = cterm_c(tm);
+      cx<T> v = cscale(coef[remap ? remap[c] : c], (T)cterm_s(tm));
 #pragma unroll
-    for (int q = 0; q < DEG; ++q) v = cmul(v, sx[term_var(tm, q)]);
-    acc = cadd(acc, v);
+      for (int q = 0; q < DEG; ++q) v = cmul(v, sx[cterm_var(tm, q)]);
+      acc = cadd(acc, v);
+    }
+  } else {
+    const Term* tp = S.ht + gle;
+#pragma unroll 2
+    for (int k = 0; k < S.htT; ++k) {
+      const Term tm = tp[(size_t)k * n];
+      cx<T> v = cscale(coef[remap ? remap[tm.c] : tm.c], (T)tm.s);
+#pragma unroll
+      for (int q = 0; q < DEG; ++q) v = cmul(v, sx[term_var(tm, q)]);
+      acc = cadd(acc, v);
+    }
   }
   return acc;
 }
@@ -150,6 +179,7 @@ __device__ __forceinline__ void stage_coeffs(const Seg& S, const Group& g, bool 
   const double2* ht = S.cht + (size_t)inst * C1 * M;
   if (M == 1) {
     // c(t) = c0 + t c1, dc/dt = Ht[j][0]
+#pragma unroll 4
     for (int j = g.gl; j < C1; j += g.G) {
       const int q = S.cslot[j];
       const double2 a = hx[2 * (size_t)j], b = hx[2 * (size_t)j + 1];
@@ -294,9 +324,10 @@ struct Lds {
 // coefficients c(t) must already be in the group's LDS.  `run` enters as the
 // set of groups to correct; ok / inf return the reference's r_isSuccessful /
 // r_isInfFail.
-template <int NMAX, int DEG, class T>
+template <int NMAX, int DEG, class T, bool LT>
 __device__ __forceinline__ void newton_pass(const Seg& S, const Opts& o, const Group& g, const Lds<NMAX>& L,
-                                            bool run, bool row, cx<double>& x, int& nnewt, bool& ok, bool& inf) {
+                                            const uint2* lt, bool run, bool row, cx<double>& x, int& nnewt, bool& ok,
+                                            bool& inf) {
   ok = false;
   inf = false;
   const int n = S.n;
@@ -308,8 +339,8 @@ __device__ __forceinline__ void newton_pass(const Seg& S, const Opts& o, const G
     publish_x<T>(sx, x, g, n, run, row);
     HCX_WAVE_SYNC();
     cx<T> A[NMAX];
-    eval_jac_row<NMAX, DEG, T>(S, gle, sx, sc, A);
-    const cx<T> h = eval_t_row<DEG, T>(S, gle, sx, sc);
+    eval_jac_row<NMAX, DEG, T, LT>(S, gle, sx, sc, A, lt);
+    const cx<T> h = eval_t_row<DEG, T, LT>(S, gle, sx, sc, lt);
     bool sing;
     const cx<T> d = group_lu_solve<NMAX, T>(A, h, g, n, row, L.template piv<T>(), sing);
     if (run && row) x = csub(x, ccast<double>(d));
@@ -335,10 +366,10 @@ using namespace dev;
 // (x0 + dt k3, t + dt); returns k1 + 2 k2 + 2 k3 + k4 (component gl).
 // sd_ext: dc/dt table of the group's instance in global memory (constant in t,
 // M = 1) or nullptr to stage dc/dt(t) in LDS next to c(t).
-template <int NMAX, int DEG, class T>
-__device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, const Lds<NMAX>& L, bool act, bool row,
-                                              int gle, int inst, double ts, double dt, cx<double> xl,
-                                              const cx<T>* sd_ext) {
+template <int NMAX, int DEG, class T, bool LT>
+__device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, const Lds<NMAX>& L, const uint2* lt,
+                                              bool act, bool row, int gle, int inst, double ts, double dt,
+                                              cx<double> xl, const cx<T>* sd_ext) {
   const int n = S.n;
   const double h = 0.5 * dt;
   cx<double> acc = cmk<double>(0, 0), kprev = cmk<double>(0, 0);
@@ -353,8 +384,8 @@ __device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, cons
     publish_x<T>(sx, xs, g, n, act, row);
     HCX_WAVE_SYNC();
     cx<T> A[NMAX];
-    eval_jac_row<NMAX, DEG, T>(S, gle, sx, sc, A);
-    const cx<T> bt = eval_t_row<DEG, T>(S, gle, sx, sd, sd_ext ? S.cinv : nullptr);
+    eval_jac_row<NMAX, DEG, T, LT>(S, gle, sx, sc, A, lt);
+    const cx<T> bt = eval_t_row<DEG, T, LT>(S, gle, sx, sd, lt, sd_ext ? S.cinv : nullptr);
     bool sing;
     const cx<T> k = group_lu_solve<NMAX, T>(A, cmk<T>(-bt.re, -bt.im), g, n, row, L.template piv<T>(), sing);
     kprev = ccast<double>(k);
@@ -385,7 +416,20 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
   const int n = S.n;
   const bool row = g.in_grp && g.gl < n;
   const int gle = row ? g.gl : 0;
-  Lds<NMAX> L{smem + (size_t)g.grp * S.lds_group};
+  // Size-specialised instances up to n = 16 read the segment's tables from a
+  // compact copy in LDS (at the start of the dynamic LDS, LN.tab_lds bytes)
+  // instead of global memory: the term loads head every evaluation's
+  // dependency chain, and LDS returns them several times sooner.
+  constexpr bool LT = EXACT && NMAX <= 16;
+  const uint2* lt = nullptr;
+  if constexpr (LT) {
+    uint2* d = reinterpret_cast<uint2*>(smem);
+    const int words = (S.hxT * n + S.htT) * n;
+    for (int i = lane; i < words; i += 64) d[i] = S.ctab[i];
+    HCX_WAVE_SYNC();
+    lt = d;
+  }
+  Lds<NMAX> L{smem + LN.tab_lds + (size_t)g.grp * S.lds_group};
 
   // per-group path state, replicated over the group's lanes
   long long path = -1;
@@ -479,16 +523,22 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
       const double ts = t;
       cx<double> acc;
       if (MODE == 0) {
-        acc = predict<NMAX, DEG, double>(S, g, L, stepping, row, gle, inst, ts, dt, xl, nullptr);
+        // dc/dt (constant when M = 1) from the instance's table in global
+        // memory halves the group's LDS and raises occupancy; a launch of few
+        // paths (latency bound) stages it in LDS instead
+        const cx<double>* sd_ext = (S.M == 1 && !LN.dot_lds)
+                                       ? reinterpret_cast<const cx<double>*>(S.cht + (size_t)inst * (S.C + 1))
+                                       : nullptr;
+        acc = predict<NMAX, DEG, double, LT>(S, g, L, lt, stepping, row, gle, inst, ts, dt, xl, sd_ext);
       } else {
         const bool p32 = stepping && !esc, p64 = stepping && esc;
         cx<double> a32 = cmk<double>(0, 0), a64 = a32;
-        if (__any(p32)) a32 = predict<NMAX, DEG, float>(S, g, L, p32, row, gle, inst, ts, dt, xl, nullptr);
+        if (__any(p32)) a32 = predict<NMAX, DEG, float, LT>(S, g, L, lt, p32, row, gle, inst, ts, dt, xl, nullptr);
         if (__any(p64)) {
           HCX_WAVE_SYNC();
           const cx<double>* sd_ext =
               S.M == 1 ? reinterpret_cast<const cx<double>*>(S.cht + (size_t)inst * (S.C + 1)) : nullptr;
-          a64 = predict<NMAX, DEG, double>(S, g, L, p64, row, gle, inst, ts, dt, xl, sd_ext);
+          a64 = predict<NMAX, DEG, double, LT>(S, g, L, lt, p64, row, gle, inst, ts, dt, xl, sd_ext);
         }
         acc = esc ? a64 : a32;
       }
@@ -500,15 +550,15 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
       // Newton corrector at the new t (:189-223)
       bool ok = false, inf = false;
       if (MODE == 0) {
-        newton_pass<NMAX, DEG, double>(S, o, g, L, stepping, row, x, nnewt, ok, inf);
+        newton_pass<NMAX, DEG, double, LT>(S, o, g, L, lt, stepping, row, x, nnewt, ok, inf);
       } else {
         const bool run32 = stepping && !ez && !esc && o.newton32, run64 = stepping && !run32;
         bool ok32 = false, inf32 = false, ok64 = false, inf64 = false;
-        if (__any(run32)) newton_pass<NMAX, DEG, float>(S, o, g, L, run32, row, x, nnewt, ok32, inf32);
+        if (__any(run32)) newton_pass<NMAX, DEG, float, LT>(S, o, g, L, lt, run32, row, x, nnewt, ok32, inf32);
         if (__any(run64)) {
           HCX_WAVE_SYNC();
           stage_coeffs<double>(S, g, run64, inst, t, L.coef(), false);
-          newton_pass<NMAX, DEG, double>(S, o, g, L, run64, row, x, nnewt, ok64, inf64);
+          newton_pass<NMAX, DEG, double, LT>(S, o, g, L, lt, run64, row, x, nnewt, ok64, inf64);
         }
         ok = run64 ? ok64 : ok32;
         inf = run64 ? inf64 : inf32;
@@ -556,8 +606,8 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
           publish_x<double>(sx, x, g, n, run, row);
           HCX_WAVE_SYNC();
           cx<double> A[NMAX];
-          eval_jac_row<NMAX, DEG, double>(S, gle, sx, sc, A);
-          const cx<double> hv = eval_t_row<DEG, double>(S, gle, sx, sc);
+          eval_jac_row<NMAX, DEG, double, LT>(S, gle, sx, sc, A, lt);
+          const cx<double> hv = eval_t_row<DEG, double, LT>(S, gle, sx, sc, lt);
           bool sing;
           const cx<double> d = group_lu_solve<NMAX, double>(A, hv, g, n, row, L.template piv<double>(), sing);
           if (run && row) x = csub(x, d);
@@ -578,7 +628,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
         // residual max_i |H_i(x, 1)|
         publish_x<double>(sx, x, g, n, ref, row);
         HCX_WAVE_SYNC();
-        const cx<double> hv = eval_t_row<DEG, double>(S, gle, sx, sc);
+        const cx<double> hv = eval_t_row<DEG, double, LT>(S, gle, sx, sc, lt);
         double r = row ? sqrt(cnorm2(hv)) : 0.0;
         if (r != r) r = INFINITY;
         resid = group_max(r, g);

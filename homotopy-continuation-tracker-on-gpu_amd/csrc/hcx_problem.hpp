@@ -17,6 +17,8 @@ struct hcx_problem {
   int* d_hx_cnt = nullptr;
   int* d_cslot = nullptr;              // [C+1] LDS slot of each coefficient
   int* d_cinv = nullptr;               // [C+1] coefficient of each slot
+  uint2* d_ctab = nullptr;             // compact hx + ht tables (8-byte terms), or null
+  int ctab_bytes = 0;                  // their size rounded up to 16 bytes
   double2* d_start = nullptr;
   double2* d_affine = nullptr;
   double2* d_p0 = nullptr;

@@ -341,14 +341,15 @@ Opts to_opts(const hcx_track_opts* o) {
 }
 
 // Per-group LDS: x and pivot-row buffers (FP64 capacity) and the coefficient
-// area: FP64 c + dc/dt (mode FP64, or mixed with M > 1 for the FP64
-// re-track), else FP32 c + dc/dt overlaid by FP64 c.
+// area: with M = 1 (dc/dt constant, read by the FP64 stages from the
+// instance's table in global memory) FP32 c + dc/dt overlaid by FP64 c, else
+// (or dot_lds: FP64 stages with dc/dt in LDS) FP64 c + dc/dt.
 // The region stride is padded to 16 mod 256 bytes: the groups of a wave read
 // the same offsets of their regions together (pivot-row broadcasts, x and
 // coefficient gathers), and this stride spreads them over distinct LDS banks
 // (64 banks x 4 bytes) for 8- and 16-byte accesses.
-int lds_group_bytes(int nmax, int C, int mode, int M) {
-  const int per = (mode == HCX_PREC_FP64 || M > 1) ? 32 : 16;
+int lds_group_bytes(int nmax, int C, int M, bool dot_lds = false) {
+  const int per = (M > 1 || dot_lds) ? 32 : 16;
   int b = 2 * 16 * (nmax + 1) + (C + 1) * per;
   b = (b + 15) & ~15;
   return ((b - 16 + 255) / 256) * 256 + 16;
@@ -357,27 +358,40 @@ int lds_group_bytes(int nmax, int C, int mode, int M) {
 struct Plan {
   int nmax = 0, mode = 0;
   bool exact = false;
+  int tab_lds = 0;   // LDS bytes of the compact tables (instances that hold them there)
   size_t lds = 0;
   KernelFn fn = nullptr;
 };
 
+// Largest compact table the size-specialised instances up to n = 16 stage in
+// LDS; a problem with larger tables runs on the generic instance.
+constexpr int kMaxTabLds = 32 * 1024;
+
 // Kernel instance for a launch: the size-specialised one when every segment
-// has the same n, runs one lane per row (G = n) and fits DEG <= 3; otherwise
-// the generic instance with the smallest NMAX >= max n.
-hcx_status choose_kernel(int nseg, const int* ns, const int* Gs, int max_deg, int mode, int specialize, Plan& pl) {
+// has the same n, runs one lane per row (G = n) and fits DEG <= 3 (and, for
+// n <= 16, has compact tables of at most kMaxTabLds bytes); otherwise the
+// generic instance with the smallest NMAX >= max n.
+hcx_status choose_kernel(int nseg, const hcx_problem* const* probs, const int* Gs, int max_deg, int mode,
+                         int specialize, Plan& pl) {
   pl.mode = mode;
-  int max_n = 1;
-  bool same = true;
+  pl.tab_lds = 0;
+  int max_n = 1, tab = 0;
+  bool same = true, tab_ok = true;
   for (int s = 0; s < nseg; ++s) {
-    max_n = std::max(max_n, ns[s]);
-    same = same && ns[s] == ns[0] && Gs[s] == ns[s];
+    const int n = probs[s]->n;
+    max_n = std::max(max_n, n);
+    same = same && n == probs[0]->n && Gs[s] == n;
+    tab_ok = tab_ok && probs[s]->d_ctab && probs[s]->ctab_bytes <= kMaxTabLds;
+    tab = std::max(tab, probs[s]->ctab_bytes);
   }
-  if (specialize && same) {
-    KernelFn fn = pick_exact(ns[0], max_deg, mode);
+  const int n0 = probs[0]->n;
+  if (specialize && same && (n0 > 16 || tab_ok)) {
+    KernelFn fn = pick_exact(n0, max_deg, mode);
     if (fn) {
       pl.fn = fn;
-      pl.nmax = ns[0];
+      pl.nmax = n0;
       pl.exact = true;
+      pl.tab_lds = n0 <= 16 ? tab : 0;
       return HCX_OK;
     }
   }
@@ -410,24 +424,27 @@ void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, 
   S.hxT = p->hxT;
   S.htT = p->htT;
   S.G = G;
-  S.lds_group = lds_group_bytes(nmax, p->C, mode, p->M);
+  S.lds_group = lds_group_bytes(nmax, p->C, p->M);
   S.hx = p->d_hx;
   S.ht = p->d_ht;
   S.hx_cnt = p->d_hx_cnt;
   S.cslot = p->d_cslot;
   S.cinv = p->d_cinv;
+  S.ctab = p->d_ctab;
   S.start = p->d_start;
   S.start_stride = p->n;
   S.counter = p->d_counter;
 }
 
-hcx_status finish_plan(const Launch& LN, Plan& pl) {
+hcx_status finish_plan(Launch& LN, Plan& pl) {
   size_t lds = 0;
   for (int s = 0; s < LN.nseg; ++s) {
     const Seg& S = LN.seg[s];
     const int groups = (64 + S.G - 1) / S.G;
     lds = std::max(lds, (size_t)groups * (size_t)S.lds_group);
   }
+  LN.tab_lds = pl.tab_lds;
+  lds += (size_t)pl.tab_lds;
   pl.lds = lds;
   if (lds > 65536) {
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(pl.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -439,9 +456,9 @@ hcx_status finish_plan(const Launch& LN, Plan& pl) {
 // Default lanes per path: n (several paths per wave, G need not divide 64),
 // widened while the wave's groups would need more than 64 KiB of LDS (wide
 // coefficient tables), so that at least two waves share a CU.
-int default_lanes(int n, int C, int M, int mode) {
+int default_lanes(int n, int C, int M) {
   const int ni = round_up_idx(kNmaxList, 3, n);
-  const int lg = lds_group_bytes(ni < 0 ? n : kNmaxList[ni], C, mode, M);
+  const int lg = lds_group_bytes(ni < 0 ? n : kNmaxList[ni], C, M);
   int G = n;
   while (G < 64 && (size_t)((64 + G - 1) / G) * (size_t)lg > 65536) ++G;
   return G;
@@ -604,6 +621,21 @@ hcx_status hcx_problem_create(const hcx_problem_desc* d, hcx_problem** out) {
   for (auto& tm : ht) tm.c = cslot[tm.c];
   for (auto& tm : hx) tm.c = cslot[tm.c];
 
+  // compact 8-byte terms (hx part, then ht part) for the instances that hold
+  // the tables in LDS: slot and scalar in 16 bits each, <= 4 variables
+  std::vector<uint2> ctab;
+  {
+    bool fits = C <= 0xffff && std::max(deg, 2) <= 4;
+    auto pack = [&](const dev::Term& tm) {
+      if (tm.s < -32768 || tm.s > 32767) fits = false;
+      return make_uint2(((uint32_t)tm.c & 0xffffu) | ((uint32_t)tm.s << 16), tm.v03);
+    };
+    ctab.reserve(hx.size() + ht.size());
+    for (const auto& tm : hx) ctab.push_back(pack(tm));
+    for (const auto& tm : ht) ctab.push_back(pack(tm));
+    if (!fits) ctab.clear();
+  }
+
   auto* p = new hcx_problem();
   (void)hipGetDevice(&p->device);
   p->n = n;
@@ -631,6 +663,12 @@ hcx_status hcx_problem_create(const hcx_problem_desc* d, hcx_problem** out) {
     return fail(e, "copy cslot");
   if ((e = hipMemcpy(p->d_cinv, cinv.data(), (C + 1) * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(e, "copy cinv");
+  if (!ctab.empty()) {
+    p->ctab_bytes = (int)((ctab.size() * sizeof(uint2) + 15) / 16 * 16);
+    if ((e = hipMalloc(&p->d_ctab, p->ctab_bytes)) != hipSuccess) return fail(e, "hipMalloc ctab");
+    if ((e = hipMemcpy(p->d_ctab, ctab.data(), ctab.size() * sizeof(uint2), hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(e, "copy ctab");
+  }
   if ((e = hipMalloc(&p->d_start, (size_t)p->K * n * sizeof(double2))) != hipSuccess) return fail(e, "hipMalloc start");
   if ((e = hipMalloc(&p->d_counter, 64 * sizeof(unsigned int))) != hipSuccess) return fail(e, "hipMalloc counter");
   if ((e = hipMalloc(&p->d_out, sizeof(SegOut))) != hipSuccess) return fail(e, "hipMalloc out");
@@ -716,6 +754,7 @@ hcx_status hcx_problem_destroy(hcx_problem* p) {
   if (p->d_hx_cnt) (void)hipFree(p->d_hx_cnt);
   if (p->d_cslot) (void)hipFree(p->d_cslot);
   if (p->d_cinv) (void)hipFree(p->d_cinv);
+  if (p->d_ctab) (void)hipFree(p->d_ctab);
   if (p->d_start) (void)hipFree(p->d_start);
   if (p->d_affine) (void)hipFree(p->d_affine);
   if (p->d_p0) (void)hipFree(p->d_p0);
@@ -794,7 +833,7 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
     ns[s] = probs[s]->n;
     if (o->lanes_per_path > 0) Gs[s] = o->lanes_per_path;
     else if (probs[s]->tuned) Gs[s] = probs[s]->tuned_G;
-    else Gs[s] = default_lanes(probs[s]->n, probs[s]->C, probs[s]->M, o->precision);
+    else Gs[s] = default_lanes(probs[s]->n, probs[s]->C, probs[s]->M);
     if (Gs[s] < ns[s]) {
       set_error("track: lanes_per_path=%d is below n=%d", Gs[s], ns[s]);
       return HCX_ERR_INVALID_ARG;
@@ -802,7 +841,7 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
   }
   if (all_tuned && nseg == 1) specialize = probs[0]->tuned_spec;
   Plan pl;
-  st = choose_kernel(nseg, ns, Gs, max_deg, o->precision, specialize, pl);
+  st = choose_kernel(nseg, probs, Gs, max_deg, o->precision, specialize, pl);
   if (st != HCX_OK) return st;
   const int nmax = pl.nmax;
   long long total_groups_needed = 0;
@@ -956,9 +995,17 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
     }
     HIP_TRY(hipGetLastError());
     Plan p2;
-    st = choose_kernel(nseg, ns, Gs, max_deg, HCX_PREC_FP64, specialize, p2);
+    st = choose_kernel(nseg, probs, Gs, max_deg, HCX_PREC_FP64, specialize, p2);
     if (st != HCX_OK) return st;
-    for (int s = 0; s < nseg; ++s) L2.seg[s].lds_group = lds_group_bytes(p2.nmax, probs[s]->C, HCX_PREC_FP64, probs[s]->M);
+    // few paths: latency bound, so dc/dt is staged in LDS when the groups'
+    // regions stay within 64 KiB
+    size_t need = 0;
+    for (int s = 0; s < nseg; ++s)
+      need = std::max(need, (size_t)((64 + L2.seg[s].G - 1) / L2.seg[s].G) *
+                                (size_t)lds_group_bytes(p2.nmax, probs[s]->C, probs[s]->M, true));
+    L2.dot_lds = need + (size_t)p2.tab_lds <= 65536 ? 1 : 0;
+    for (int s = 0; s < nseg; ++s)
+      L2.seg[s].lds_group = lds_group_bytes(p2.nmax, probs[s]->C, probs[s]->M, L2.dot_lds != 0);
     st = finish_plan(L2, p2);
     if (st != HCX_OK) return st;
     // a small persistent grid: the list holds few paths; idle waves exit at once
@@ -1028,9 +1075,9 @@ double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, 
   LN.nseg = 1;
   LN.persistent = 0;
   LN.o = to_opts(&o);
-  const int G = default_lanes(p->n, p->C, p->M, HCX_PREC_FP64);
+  const int G = default_lanes(p->n, p->C, p->M);
   Plan pl;
-  if (choose_kernel(1, &p->n, &G, p->deg, HCX_PREC_FP64, 1, pl) != HCX_OK) return -1.0;
+  if (choose_kernel(1, &p, &G, p->deg, HCX_PREC_FP64, 1, pl) != HCX_OK) return -1.0;
   Seg& S = LN.seg[0];
   SegOut& O = OP.out[0];
   seg_from_problem(S, O, p, G, HCX_PREC_FP64, pl.nmax);
@@ -1154,7 +1201,7 @@ hcx_status hcx_autotune(hcx_problem* p, const hcx_batch_desc* sample, const hcx_
   std::memset(&r, 0, sizeof(r));
   r.d_roots = reinterpret_cast<hcx_complex*>(roots);
   r.d_flags = flags;
-  const int G0 = default_lanes(p->n, p->C, p->M, o->precision);
+  const int G0 = default_lanes(p->n, p->C, p->M);
   int pow2 = 1;
   while (pow2 < G0) pow2 <<= 1;
   struct Cand { int G, spec, wpc; };
@@ -1164,7 +1211,7 @@ hcx_status hcx_autotune(hcx_problem* p, const hcx_batch_desc* sample, const hcx_
     // occupancy of the default shape, to derive the half-density candidate
     Plan pl;
     int G = G0;
-    if (choose_kernel(1, &p->n, &G, p->deg, o->precision, 1, pl) == HCX_OK) {
+    if (choose_kernel(1, &p, &G, p->deg, o->precision, 1, pl) == HCX_OK) {
       Launch LN;
       std::memset(&LN, 0, sizeof(LN));
       SegOut O;
