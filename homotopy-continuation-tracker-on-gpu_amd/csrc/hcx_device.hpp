@@ -118,6 +118,30 @@ __device__ __forceinline__ cx<T> lane_read(cx<T> v, int src) {
   return cmk<T>(lane_read(v.re, src), lane_read(v.im, src));
 }
 
+// DPP lane permutations (a VALU operand modifier, no LDS round trip) inside
+// 16-lane rows: quad_perm xor 1 and xor 2, half-row mirror (lane i <-> 7 - i
+// of each 8), row mirror (lane i <-> 15 - i).  Composed in this order they
+// reach every lane of an aligned group of 2, 4, 8 or 16 lanes.
+constexpr int kDppQuadXor1 = 0xB1;
+constexpr int kDppQuadXor2 = 0x4E;
+constexpr int kDppHalfMirror = 0x141;
+constexpr int kDppRowMirror = 0x140;
+template <int CTRL>
+__device__ __forceinline__ int dpp_read(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_read(float v) {
+  return __int_as_float(dpp_read<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_read(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_read<CTRL>((int)(b & 0xffffffffll));
+  const int hi = dpp_read<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Orders LDS traffic between the phases of a wave.  Kernels run with
 // 64-thread workgroups (one wave); a wave's LDS instructions execute in issue
 // order, so a wave-scope fence (no memory-counter drain) is enough to keep the

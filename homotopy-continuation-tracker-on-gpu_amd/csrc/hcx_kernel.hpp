@@ -16,6 +16,12 @@ using dev::cmk;
 constexpr int kMaxSeg = 8;
 constexpr int kMaxN = 32;
 
+// Lanes per path of the size-specialised instance for n: n, except n = 30,
+// which runs on aligned 32-lane groups (still two paths per wave) so that
+// its group reductions can use DPP row permutations.
+constexpr int exact_lanes(int n) { return n == 30 ? 32 : n; }
+constexpr bool is_pow2(int g) { return g > 0 && (g & (g - 1)) == 0; }
+
 // Output pointers of a segment (cold): kept in device memory and read only
 // when a path finishes, so they do not occupy scalar registers in the loop.
 struct SegOut {
@@ -233,13 +239,45 @@ __device__ __forceinline__ T group_max(T v, const Group& g) {
   return v;
 }
 
+// Argmax step of the pivot search: keep the larger candidate, ties to the
+// smaller key (position label << 6 | lane).
+template <class T>
+__device__ __forceinline__ void piv_take(T& v, int& key, T ov, int ok) {
+  const bool take = (ov > v) | ((ov == v) & (ok < key));
+  v = take ? ov : v;
+  key = take ? ok : key;
+}
+
+// Pivot argmax over the group: GC = compile-time lanes per path when that is
+// a power of two (aligned groups; DPP inside 16-lane rows, one lane exchange
+// per further doubling), else 0 (ring of ceil(log2 G) lane exchanges).
+template <int GC, class T>
+__device__ __forceinline__ void group_argmax(T& v, int& key, const Group& g) {
+  if constexpr (GC > 0) {
+    static_assert(is_pow2(GC) && GC <= 64, "aligned power-of-two groups");
+    if constexpr (GC >= 2) piv_take(v, key, dpp_read<kDppQuadXor1>(v), dpp_read<kDppQuadXor1>(key));
+    if constexpr (GC >= 4) piv_take(v, key, dpp_read<kDppQuadXor2>(v), dpp_read<kDppQuadXor2>(key));
+    if constexpr (GC >= 8) piv_take(v, key, dpp_read<kDppHalfMirror>(v), dpp_read<kDppHalfMirror>(key));
+    if constexpr (GC >= 16) piv_take(v, key, dpp_read<kDppRowMirror>(v), dpp_read<kDppRowMirror>(key));
+    const int lane = g.gbase + g.gl;
+    if constexpr (GC >= 32) piv_take(v, key, lane_read(v, lane ^ 16), lane_read(key, lane ^ 16));
+    if constexpr (GC >= 64) piv_take(v, key, lane_read(v, lane ^ 32), lane_read(key, lane ^ 32));
+  } else {
+    for (int s = 1; s < g.G; s <<= 1) {
+      int src = g.gl + s;
+      src = (src >= g.G ? src - g.G : src) + g.gbase;
+      piv_take(v, key, lane_read(v, src), lane_read(key, src));
+    }
+  }
+}
+
 // ------------------------------------------------------------ group LU
 // Solves A x = b for the group's n x n system, row r held by lane r (A[j] =
 // A[r][j]).  Returns x_gl to lane gl.  Partial pivoting on cabs1 with the
 // first maximum in current row order (rows carry a position label that is
 // swapped like a physical row interchange); a zero pivot column is skipped
 // and reported through `singular` (dev-cgesv-batched-small.cuh:51-95).
-template <int NMAX, class T>
+template <int NMAX, int GC, class T>
 __device__ __forceinline__ cx<T> group_lu_solve(cx<T> (&A)[NMAX], cx<T> b, const Group& g, int n, bool row,
                                                 cx<T>* sp, bool& singular) {
   int label = g.gl;
@@ -254,19 +292,11 @@ __device__ __forceinline__ cx<T> group_lu_solve(cx<T> (&A)[NMAX], cx<T> b, const
       T v = (row && label >= k) ? cabs1(A[k]) : T(-1);
       if (v != v) v = T(INFINITY);
       int key = (label << 6) | g.gl;
-      for (int s = 1; s < g.G; s <<= 1) {
-        int src = g.gl + s;
-        src = (src >= g.G ? src - g.G : src) + g.gbase;
-        const T ov = lane_read(v, src);
-        const int ok = lane_read(key, src);
-        const bool take = (ov > v) | ((ov == v) & (ok < key));
-        v = take ? ov : v;
-        key = take ? ok : key;
-      }
+      group_argmax<GC>(v, key, g);
       const int bl = key & 63, blabel = key >> 6;
       const bool zero = !(v > T(0));
       singular = singular || zero;
-      if (g.gl == bl) {
+      if (g.in_grp && g.gl == bl) {
 #pragma unroll
         for (int j = k; j < NMAX; ++j)
           if (j < n) sp[j] = A[j];
@@ -324,7 +354,7 @@ struct Lds {
 // coefficients c(t) must already be in the group's LDS.  `run` enters as the
 // set of groups to correct; ok / inf return the reference's r_isSuccessful /
 // r_isInfFail.
-template <int NMAX, int DEG, class T, bool LT>
+template <int NMAX, int GC, int DEG, class T, bool LT>
 __device__ __forceinline__ void newton_pass(const Seg& S, const Opts& o, const Group& g, const Lds<NMAX>& L,
                                             const uint2* lt, bool run, bool row, cx<double>& x, int& nnewt, bool& ok,
                                             bool& inf) {
@@ -342,7 +372,7 @@ __device__ __forceinline__ void newton_pass(const Seg& S, const Opts& o, const G
     eval_jac_row<NMAX, DEG, T, LT>(S, gle, sx, sc, A, lt);
     const cx<T> h = eval_t_row<DEG, T, LT>(S, gle, sx, sc, lt);
     bool sing;
-    const cx<T> d = group_lu_solve<NMAX, T>(A, h, g, n, row, L.template piv<T>(), sing);
+    const cx<T> d = group_lu_solve<NMAX, GC, T>(A, h, g, n, row, L.template piv<T>(), sing);
     if (run && row) x = csub(x, ccast<double>(d));
     double rs = row ? (double)cnorm2(d) : 0.0;
     double rc = row ? cnorm2(x) : 0.0;
@@ -366,7 +396,7 @@ using namespace dev;
 // (x0 + dt k3, t + dt); returns k1 + 2 k2 + 2 k3 + k4 (component gl).
 // sd_ext: dc/dt table of the group's instance in global memory (constant in t,
 // M = 1) or nullptr to stage dc/dt(t) in LDS next to c(t).
-template <int NMAX, int DEG, class T, bool LT>
+template <int NMAX, int GC, int DEG, class T, bool LT>
 __device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, const Lds<NMAX>& L, const uint2* lt,
                                               bool act, bool row, int gle, int inst, double ts, double dt,
                                               cx<double> xl, const cx<T>* sd_ext) {
@@ -387,7 +417,7 @@ __device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, cons
     eval_jac_row<NMAX, DEG, T, LT>(S, gle, sx, sc, A, lt);
     const cx<T> bt = eval_t_row<DEG, T, LT>(S, gle, sx, sd, lt, sd_ext ? S.cinv : nullptr);
     bool sing;
-    const cx<T> k = group_lu_solve<NMAX, T>(A, cmk<T>(-bt.re, -bt.im), g, n, row, L.template piv<T>(), sing);
+    const cx<T> k = group_lu_solve<NMAX, GC, T>(A, cmk<T>(-bt.re, -bt.im), g, n, row, L.template piv<T>(), sing);
     kprev = ccast<double>(k);
     const double w = (st == 1 || st == 2) ? 2.0 : 1.0;
     acc = cmk<double>(acc.re + w * kprev.re, acc.im + w * kprev.im);
@@ -405,12 +435,15 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
   // local copies: the fields are wave-uniform and live in SGPRs (scalar
   // kernarg loads) instead of being re-read through a generic pointer
   Seg S = LN.seg[si];
+  constexpr int GE = exact_lanes(NMAX);
   if (EXACT) {
-    // size-specialised instance: n = G = NMAX are compile-time constants,
+    // size-specialised instance: n = NMAX and G are compile-time constants,
     // so every loop bound and the lane-group geometry fold away
     S.n = NMAX;
-    S.G = NMAX;
+    S.G = GE;
   }
+  // compile-time power-of-two lanes per path: DPP group reductions
+  constexpr int GC = (EXACT && is_pow2(GE)) ? GE : 0;
   const Opts o = LN.o;
   const Group g = make_group(S.G, lane);
   const int n = S.n;
@@ -429,7 +462,9 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
     HCX_WAVE_SYNC();
     lt = d;
   }
-  Lds<NMAX> L{smem + LN.tab_lds + (size_t)g.grp * S.lds_group};
+  // lanes outside the full groups (64 % G of them) alias region 0: they
+  // track nothing and store nothing to LDS
+  Lds<NMAX> L{smem + LN.tab_lds + (size_t)(g.in_grp ? g.grp : 0) * S.lds_group};
 
   // per-group path state, replicated over the group's lanes
   long long path = -1;
@@ -529,16 +564,16 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
         const cx<double>* sd_ext = (S.M == 1 && !LN.dot_lds)
                                        ? reinterpret_cast<const cx<double>*>(S.cht + (size_t)inst * (S.C + 1))
                                        : nullptr;
-        acc = predict<NMAX, DEG, double, LT>(S, g, L, lt, stepping, row, gle, inst, ts, dt, xl, sd_ext);
+        acc = predict<NMAX, GC, DEG, double, LT>(S, g, L, lt, stepping, row, gle, inst, ts, dt, xl, sd_ext);
       } else {
         const bool p32 = stepping && !esc, p64 = stepping && esc;
         cx<double> a32 = cmk<double>(0, 0), a64 = a32;
-        if (__any(p32)) a32 = predict<NMAX, DEG, float, LT>(S, g, L, lt, p32, row, gle, inst, ts, dt, xl, nullptr);
+        if (__any(p32)) a32 = predict<NMAX, GC, DEG, float, LT>(S, g, L, lt, p32, row, gle, inst, ts, dt, xl, nullptr);
         if (__any(p64)) {
           HCX_WAVE_SYNC();
           const cx<double>* sd_ext =
               S.M == 1 ? reinterpret_cast<const cx<double>*>(S.cht + (size_t)inst * (S.C + 1)) : nullptr;
-          a64 = predict<NMAX, DEG, double, LT>(S, g, L, lt, p64, row, gle, inst, ts, dt, xl, sd_ext);
+          a64 = predict<NMAX, GC, DEG, double, LT>(S, g, L, lt, p64, row, gle, inst, ts, dt, xl, sd_ext);
         }
         acc = esc ? a64 : a32;
       }
@@ -550,15 +585,15 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
       // Newton corrector at the new t (:189-223)
       bool ok = false, inf = false;
       if (MODE == 0) {
-        newton_pass<NMAX, DEG, double, LT>(S, o, g, L, lt, stepping, row, x, nnewt, ok, inf);
+        newton_pass<NMAX, GC, DEG, double, LT>(S, o, g, L, lt, stepping, row, x, nnewt, ok, inf);
       } else {
         const bool run32 = stepping && !ez && !esc && o.newton32, run64 = stepping && !run32;
         bool ok32 = false, inf32 = false, ok64 = false, inf64 = false;
-        if (__any(run32)) newton_pass<NMAX, DEG, float, LT>(S, o, g, L, lt, run32, row, x, nnewt, ok32, inf32);
+        if (__any(run32)) newton_pass<NMAX, GC, DEG, float, LT>(S, o, g, L, lt, run32, row, x, nnewt, ok32, inf32);
         if (__any(run64)) {
           HCX_WAVE_SYNC();
           stage_coeffs<double>(S, g, run64, inst, t, L.coef(), false);
-          newton_pass<NMAX, DEG, double, LT>(S, o, g, L, lt, run64, row, x, nnewt, ok64, inf64);
+          newton_pass<NMAX, GC, DEG, double, LT>(S, o, g, L, lt, run64, row, x, nnewt, ok64, inf64);
         }
         ok = run64 ? ok64 : ok32;
         inf = run64 ? inf64 : inf32;
@@ -609,7 +644,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
           eval_jac_row<NMAX, DEG, double, LT>(S, gle, sx, sc, A, lt);
           const cx<double> hv = eval_t_row<DEG, double, LT>(S, gle, sx, sc, lt);
           bool sing;
-          const cx<double> d = group_lu_solve<NMAX, double>(A, hv, g, n, row, L.template piv<double>(), sing);
+          const cx<double> d = group_lu_solve<NMAX, GC, double>(A, hv, g, n, row, L.template piv<double>(), sing);
           if (run && row) x = csub(x, d);
           double nd = row ? cnorm2(d) : 0.0, nx = row ? cnorm2(x) : 0.0;
           nd = group_sum(nd, g, lane);

@@ -368,7 +368,8 @@ struct Plan {
 constexpr int kMaxTabLds = 32 * 1024;
 
 // Kernel instance for a launch: the size-specialised one when every segment
-// has the same n, runs one lane per row (G = n) and fits DEG <= 3 (and, for
+// has the same n, runs its lanes per path (G = exact_lanes(n)) and fits
+// DEG <= 3 (and, for
 // n <= 16, has compact tables of at most kMaxTabLds bytes); otherwise the
 // generic instance with the smallest NMAX >= max n.
 hcx_status choose_kernel(int nseg, const hcx_problem* const* probs, const int* Gs, int max_deg, int mode,
@@ -380,7 +381,7 @@ hcx_status choose_kernel(int nseg, const hcx_problem* const* probs, const int* G
   for (int s = 0; s < nseg; ++s) {
     const int n = probs[s]->n;
     max_n = std::max(max_n, n);
-    same = same && n == probs[0]->n && Gs[s] == n;
+    same = same && n == probs[0]->n && Gs[s] == exact_lanes(n);
     tab_ok = tab_ok && probs[s]->d_ctab && probs[s]->ctab_bytes <= kMaxTabLds;
     tab = std::max(tab, probs[s]->ctab_bytes);
   }
@@ -440,7 +441,9 @@ hcx_status finish_plan(Launch& LN, Plan& pl) {
   size_t lds = 0;
   for (int s = 0; s < LN.nseg; ++s) {
     const Seg& S = LN.seg[s];
-    const int groups = (64 + S.G - 1) / S.G;
+    // one region per full group; the lanes of a partial group share
+    // region 0 and never write LDS
+    const int groups = 64 / S.G;
     lds = std::max(lds, (size_t)groups * (size_t)S.lds_group);
   }
   LN.tab_lds = pl.tab_lds;
@@ -453,14 +456,15 @@ hcx_status finish_plan(Launch& LN, Plan& pl) {
   return HCX_OK;
 }
 
-// Default lanes per path: n (several paths per wave, G need not divide 64),
-// widened while the wave's groups would need more than 64 KiB of LDS (wide
-// coefficient tables), so that at least two waves share a CU.
+// Default lanes per path: those of the size-specialised instance (n, or 32
+// for n = 30; several paths per wave, G need not divide 64), widened while
+// the wave's groups would need more than 64 KiB of LDS (wide coefficient
+// tables), so that at least two waves share a CU.
 int default_lanes(int n, int C, int M) {
   const int ni = round_up_idx(kNmaxList, 3, n);
   const int lg = lds_group_bytes(ni < 0 ? n : kNmaxList[ni], C, M);
-  int G = n;
-  while (G < 64 && (size_t)((64 + G - 1) / G) * (size_t)lg > 65536) ++G;
+  int G = exact_lanes(n);
+  while (G < 64 && (size_t)(64 / G) * (size_t)lg > 65536) ++G;
   return G;
 }
 
@@ -1001,7 +1005,7 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
     // regions stay within 64 KiB
     size_t need = 0;
     for (int s = 0; s < nseg; ++s)
-      need = std::max(need, (size_t)((64 + L2.seg[s].G - 1) / L2.seg[s].G) *
+      need = std::max(need, (size_t)(64 / L2.seg[s].G) *
                                 (size_t)lds_group_bytes(p2.nmax, probs[s]->C, probs[s]->M, true));
     L2.dot_lds = need + (size_t)p2.tab_lds <= 65536 ? 1 : 0;
     for (int s = 0; s < nseg; ++s)
