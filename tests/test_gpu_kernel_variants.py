@@ -1,0 +1,79 @@
+"""Kernel instances that must agree bit for bit.
+
+The size-specialised instances read compact term tables from LDS (n <= 16)
+and reduce pivot candidates with DPP permutations (power-of-two lanes per
+path, n = 30 on 32-lane groups); the generic instances read the full tables
+from global memory and reduce with lane exchanges.  Both evaluate the same
+terms in the same order and pick the same pivots, so with the same lanes per
+path every output (flags, step and Newton counts, roots) is identical.  With
+different lanes per path only the group sums of the Newton norms change their
+order, so roots agree to the parity tolerance instead."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _type(hcx, n, K, seed=5):
+    from hcx import workloads
+    ty = workloads.quadratic_types([(n, K)], seed=seed)[0]
+    prob = hcx.Problem.from_system(ty["system"], ty["start"], p0=ty["p0"])
+    return ty, prob
+
+
+def _run(hcx, prob, batch, **kw):
+    import torch
+    r = hcx.track(prob, batch, hcx.default_opts(**kw))
+    torch.cuda.synchronize()
+    return r.to_host()
+
+
+def _assert_identical(a, b, label):
+    print(f"[{label}] refined {int(((a['flags'] & 4) != 0).sum())} / {len(a['flags'])}", flush=True)
+    assert (a["flags"] == b["flags"]).all(), f"{label}: flags differ at {np.nonzero(a['flags'] != b['flags'])[0][:8]}"
+    assert (a["steps"] == b["steps"]).all(), label
+    assert (a["newton"] == b["newton"]).all(), label
+    both_nan = np.isnan(a["roots"]) & np.isnan(b["roots"])
+    assert ((a["roots"] == b["roots"]) | both_nan).all(), f"{label}: roots differ"
+
+
+@pytest.mark.parametrize("n,K,instances", [(6, 64, 16), (12, 583, 2), (16, 40, 16)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_lds_tables_equal_global_tables(hcx_mod, gpu, n, K, instances, prec):
+    """Specialised instance (LDS compact tables) == generic instance (global
+    tables), same lanes per path."""
+    from hcx import workloads
+    hcx = hcx_mod
+    ty, prob = _type(hcx, n, K)
+    b = hcx.Batch(prob, instances, target_params=workloads.targets(ty["k"], instances, seed=77))
+    a = _run(hcx, prob, b, precision=prec, lanes_per_path=n, specialize=1)
+    g = _run(hcx, prob, b, precision=prec, lanes_per_path=n, specialize=0)
+    _assert_identical(a, g, f"n={n} prec={prec}")
+    prob.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_n30_dpp_groups(hcx_mod, gpu, prec):
+    """n = 30: the specialised 32-lane instance (DPP argmax) equals the generic
+    instance at 32 lanes (lane-exchange argmax) exactly, and the generic
+    instance at 30 lanes to the parity tolerance (a path whose step sequence
+    changes with the summation order may end on another root, so 99% of the
+    paths refined by both must agree)."""
+    from hcx import workloads
+    hcx = hcx_mod
+    ty, prob = _type(hcx, 30, 312)
+    b = hcx.Batch(prob, 2, target_params=workloads.targets(ty["k"], 2, seed=77))
+    a = _run(hcx, prob, b, precision=prec, specialize=1)
+    g32 = _run(hcx, prob, b, precision=prec, lanes_per_path=32, specialize=0)
+    _assert_identical(a, g32, f"n=30 G=32 prec={prec}")
+    g30 = _run(hcx, prob, b, precision=prec, lanes_per_path=30, specialize=0)
+    ra, rb = (a["flags"] & 4) != 0, (g30["flags"] & 4) != 0
+    assert ra.sum() == rb.sum()
+    both = ra & rb
+    err = np.abs(a["roots"][both] - g30["roots"][both]).max(axis=1) / np.maximum(
+        1.0, np.abs(g30["roots"][both]).max(axis=1))
+    agree = int((err <= 1e-10).sum())
+    print(f"[n=30 G=30 prec={prec}] refined {int(ra.sum())}, agree {agree} / {int(both.sum())}", flush=True)
+    assert both.sum() >= 0.99 * ra.sum()
+    assert agree >= 0.99 * both.sum()
+    prob.close()
