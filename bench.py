@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--inflight", type=int, default=2,
+                    help="config 2: solves in flight on separate streams (double buffering: the "
+                         "duplicate re-track and tail of one solve overlap the next)")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="end with one 256 MiB device copy, the reference dispatch for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scale factors")
@@ -133,19 +136,35 @@ def main():
     res = hcx.Result(P, prob.n, I)
     stream = torch.cuda.current_stream()
 
-    def solve(opts, result):
-        hcx.track(prob, batch, opts, result=result, stream=stream)
-        return hcx.collect(result, opts, stream=stream)
+    # double buffering: each solve in flight has its own problem handle (launch
+    # workspace), batch (coefficient workspace), result and stream; the host
+    # enqueues solve i before it waits for the counts of solve i - 1
+    nf = max(1, args.inflight)
+    lanes = [(prob, batch, res, stream)]
+    for _ in range(nf - 1):
+        pk = hcx.Problem.from_system(sysf, start, p0=p0)
+        lanes.append((pk, hcx.Batch(pk, I, target_params=p1), hcx.Result(P, prob.n, I), torch.cuda.Stream()))
+
+    def run_steps(opts, count):
+        counts = None
+        for i in range(count):
+            pk, bk, rk, sk = lanes[i % nf]
+            hcx.track(pk, bk, opts, result=rk, stream=sk)
+            if i + 1 >= nf:
+                pj, bj, rj, sj = lanes[(i + 1 - nf) % nf]
+                counts = hcx.collect(rj, opts, stream=sj)
+        for i in range(max(0, count - nf + 1), count):
+            pj, bj, rj, sj = lanes[i % nf]
+            counts = hcx.collect(rj, opts, stream=sj)
+        return counts
 
     # ------------------------------------------------------------ mixed batched
-    log(f"bench: mixed batched, {I} instances x {K} paths, warmup {args.warmup}, steps {args.steps}")
-    for _ in range(args.warmup):
-        solve(opts_mixed, res)
+    log(f"bench: mixed batched, {I} instances x {K} paths, warmup {args.warmup}, steps {args.steps}, "
+        f"{nf} solves in flight")
+    run_steps(opts_mixed, args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    counts = None
-    for _ in range(args.steps):
-        counts = solve(opts_mixed, res)
+    counts = run_steps(opts_mixed, args.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
@@ -245,7 +264,7 @@ def main():
         "config": {"workload": "n=6 parameter homotopy, 12 monomials/eq, k=24 params, 64 start roots, "
                                f"{I} target instances ({P} paths per step)",
                    "instances": I, "paths_per_instance": K, "seed": args.seed,
-                   "lanes_per_path": args.lanes or prob.n},
+                   "lanes_per_path": args.lanes or prob.n, "solves_in_flight": nf},
         "p50_ms": percentile(lat_ms, 50),
         "p99_ms": percentile(lat_ms, 99),
         "latency_note": "per-instance completion time from batch launch (device clock)",
@@ -321,19 +340,28 @@ def bench_config1(args):
         _, hx, ht = sysf.total_degree(sysf.eval_coeffs(None), hcx.random_gamma(5000 + i))
         hxs.append(hx)
         hts.append(ht)
-    b = hcx.Batch(prob, R, coeffs_hx=np.stack(hxs), coeffs_ht=np.stack(hts))
-    r = hcx.Result(R * 8, 3, R)
+    hxs, hts = np.stack(hxs), np.stack(hts)
+    # solves in flight on separate streams, each with its own problem handle
+    # (launch workspace), batch and result
+    nf = max(1, args.inflight)
+    lanes = [(prob, hcx.Batch(prob, R, coeffs_hx=hxs, coeffs_ht=hts), hcx.Result(R * 8, 3, R), stream)]
+    for _ in range(nf - 1):
+        pk = hcx.Problem(3, nc, hxi, hti, c["start"])
+        lanes.append((pk, hcx.Batch(pk, R, coeffs_hx=hxs, coeffs_ht=hts), hcx.Result(R * 8, 3, R),
+                      torch.cuda.Stream()))
     opts = hcx.default_opts(precision=hcx.PREC_MIXED)
-    hcx.track(prob, b, opts, result=r, stream=stream)
+    for pk, bk, rk, sk in lanes:
+        hcx.track(pk, bk, opts, result=rk, stream=sk)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        hcx.track(prob, b, opts, result=r, stream=stream)
+    for i in range(args.steps):
+        pk, bk, rk, sk = lanes[i % nf]
+        hcx.track(pk, bk, opts, result=rk, stream=sk)
     torch.cuda.synchronize()
     el = (time.perf_counter() - t0) / args.steps
-    cnt = hcx.collect(r, opts, stream=stream)
+    cnt = hcx.collect(lanes[0][2], opts, stream=stream)
     out.update({"value": cnt["refined"] / el, "unit": "converged paths/s", "ms_per_step": el * 1e3,
-                "instances": R, "counts": cnt})
+                "instances": R, "solves_in_flight": nf, "counts": cnt})
     print(json.dumps(out))
 
 
@@ -555,8 +583,9 @@ def bench_config5(args):
     import hcx
     from hcx import workloads
     rate = float(os.environ.get("HCX_SERVE_RATE", "2000"))
-    duration = float(os.environ.get("HCX_SERVE_SECONDS", "20"))
+    duration = float(os.environ.get("HCX_SERVE_SECONDS", "30"))
     fixed = float(os.environ.get("HCX_SERVE_FIXED_RATE", "400"))
+    fixed_hi = float(os.environ.get("HCX_SERVE_FIXED_RATE_HIGH", "800"))
     log("config5: seeding start roots of six types")
     types = workloads.quadratic_types(workloads.CONFIG5_TYPES, seed=args.seed)
     probs = [hcx.Problem.from_system(t["system"], t["start"], p0=t["p0"]) for t in types]
@@ -568,6 +597,7 @@ def bench_config5(args):
     srv.drain()
     saturated = _serve(srv, tids, types, rate, duration, seed=args.seed)
     steady = _serve(srv, tids, types, fixed, duration, seed=args.seed + 1)
+    steady_hi = _serve(srv, tids, types, fixed_hi, duration, seed=args.seed + 2) if fixed_hi > 0 else None
     srv.close()
     log("config5: plain FP64 per-problem launches over the same mix")
     fp64 = _fp64_per_problem_mix(probs, types, per_type=24, seed=args.seed * 31)
@@ -575,7 +605,7 @@ def bench_config5(args):
            "config": {"workload": f"config 5: serving loop, Poisson arrivals, types (n,K)={workloads.CONFIG5_TYPES} "
                                   f"drawn uniformly, one instance per request, mixed FP32/FP64",
                       "hbm_budget_bytes": 4 << 30, "streams": 4, "max_batch_requests": 64},
-           "at_offered_rate": saturated, "at_fixed_rate": steady,
+           "at_offered_rate": saturated, "at_fixed_rate": steady, "at_fixed_rate_high": steady_hi,
            "p50_ms": steady["p50_ms"], "p99_ms": steady["p99_ms"], "latency_rate_req_s": fixed,
            "fp64_per_problem": fp64,
            "speedup_vs_fp64_per_problem": saturated["converged_paths_per_s"] / fp64["converged_paths_per_s"]}
