@@ -101,14 +101,16 @@ def main():
                     help="end with one 256 MiB device copy, the reference dispatch for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scale factors")
     ap.add_argument("--config", type=int, default=2,
-                    help="workload: 2 (default, the headline line), 1, 3, 4 or 5")
+                    help="workload: 2 (default, the headline line), 1, 3, 4, 5, or 6 (per-type cost "
+                         "sweep of the serving types)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.config != 2:
-        return {1: bench_config1, 3: bench_config3, 4: bench_config4, 5: bench_config5}[args.config](args)
+        return {1: bench_config1, 3: bench_config3, 4: bench_config4, 5: bench_config5,
+                6: bench_type_sweep}[args.config](args)
 
     import numpy as np
     import torch
@@ -610,6 +612,45 @@ def bench_config5(args):
            "fp64_per_problem": fp64,
            "speedup_vs_fp64_per_problem": saturated["converged_paths_per_s"] / fp64["converged_paths_per_s"]}
     print(json.dumps(out))
+
+
+def bench_type_sweep(args):
+    """GPU cost per request of each serving type: the six quadratic types
+    n in {3, 6, 9, 12, 16, 30}, one batched mixed-precision launch of
+    `--instances` requests (default 64) per type, timed over `--steps`
+    launches.  Also the workload of the rocprofv3 counter profiles."""
+    import torch
+    import hcx
+    from hcx import workloads
+    B = 64 if args.instances == 1024 else args.instances
+    types = workloads.quadratic_types(workloads.CONFIG5_TYPES, seed=5)
+    opts = hcx.default_opts(precision=hcx.PREC_MIXED)
+    stream = torch.cuda.current_stream()
+    rows = []
+    for ty in types:
+        prob = hcx.Problem.from_system(ty["system"], ty["start"], p0=ty["p0"])
+        b = hcx.Batch(prob, B, target_params=workloads.targets(ty["k"], B, seed=77))
+        r = hcx.Result(B * prob.num_tracks, prob.n, B)
+        hcx.track(prob, b, opts, result=r, stream=stream)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(stream)
+        for _ in range(args.steps):
+            hcx.track(prob, b, opts, result=r, stream=stream)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / args.steps
+        cnt = hcx.collect(r, opts, stream=stream)
+        h = r.to_host()
+        rows.append({"n": ty["n"], "K": ty["K"], "requests": B, "ms_per_launch": ms, "ms_per_request": ms / B,
+                     "converged_paths_per_s": cnt["refined"] / (ms * 1e-3), "refined": cnt["refined"],
+                     "steps_mean": float(h["steps"].mean()), "steps_max": int(h["steps"].max())})
+        log(f"sweep: n={ty['n']} K={ty['K']}: {ms:.2f} ms per {B} requests")
+        del b, r
+        prob.close()
+    print(json.dumps({"metric": METRIC, "config": {"workload": f"per-type cost sweep, {B} requests per type, "
+                                                               "mixed FP32/FP64, one batched launch each"},
+                      "types": rows}))
 
 
 if __name__ == "__main__":
