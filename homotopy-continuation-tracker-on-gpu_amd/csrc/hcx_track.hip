@@ -854,7 +854,9 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
     hcx_problem* p = probs[s];
     const hcx_batch_desc& b = batches[s];
     const hcx_result_desc& r = results[s];
-    if (b.num_instances < 0 || !r.d_roots || !r.d_flags) {
+    // an empty batch (no instances) needs no buffers and tracks nothing
+    const bool empty = b.num_instances == 0 || p->K == 0;
+    if (b.num_instances < 0 || (!empty && (!r.d_roots || !r.d_flags))) {
       set_error("track: segment %d: bad batch or result descriptor", s);
       return HCX_ERR_INVALID_ARG;
     }
@@ -864,7 +866,10 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
     seg_from_problem(S, O, p, G, o->precision, nmax);
     S.I = b.num_instances;
     S.P = (long long)b.num_instances * p->K;
-    if (b.d_coeffs_hx) {
+    if (empty) {
+      S.chx = nullptr;
+      S.cht = nullptr;
+    } else if (b.d_coeffs_hx) {
       if (!b.d_coeffs_ht) {
         set_error("track: segment %d: coefficient tables need both Hx and Ht", s);
         return HCX_ERR_INVALID_ARG;
@@ -1151,7 +1156,8 @@ hcx_status hcx_track_static(hcx_problem* p, const hcx_batch_desc* b, const hcx_t
 hcx_status hcx_collect(int n, int num_instances, int num_tracks, const hcx_complex* d_roots, uint32_t* d_flags,
                        double unique_tol, hcx_counts* out, void* stream) {
   clear_error();
-  if (n < 1 || num_instances < 0 || num_tracks < 0 || !d_roots || !d_flags || !out) {
+  const bool empty = num_instances == 0 || num_tracks == 0;
+  if (n < 1 || num_instances < 0 || num_tracks < 0 || (!empty && (!d_roots || !d_flags)) || !out) {
     set_error("hcx_collect: invalid arguments");
     return HCX_ERR_INVALID_ARG;
   }
@@ -1160,7 +1166,7 @@ hcx_status hcx_collect(int n, int num_instances, int num_tracks, const hcx_compl
   std::lock_guard<std::mutex> lk(g_mu);
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, g_counts, 5);
-  if (num_instances > 0)
+  if (!empty)
     hipLaunchKernelGGL(collect_kernel, dim3(num_instances), dim3(256), 0, s, n, num_tracks,
                        reinterpret_cast<const double2*>(d_roots), d_flags, unique_tol, g_counts);
   HIP_TRY(hipGetLastError());

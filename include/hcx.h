@@ -150,7 +150,7 @@ hcx_status hcx_problem_get_info(const hcx_problem* p, hcx_problem_info* info);
 /* one instance to a batch of instances tracked in one persistent launch.    */
 /* ------------------------------------------------------------------------ */
 typedef struct {
-  int num_instances;                 /* I */
+  int num_instances;                 /* I; 0 is a valid empty batch (no buffers needed) */
   /* Either per-instance coefficient-polynomial-in-t tables (the P2C output,
    * Data_Reader.cpp:59-89):  Hx[I][C+1][M+1] and Ht[I][C+1][M] ...           */
   const hcx_complex* d_coeffs_hx;
