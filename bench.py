@@ -96,9 +96,12 @@ def main():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed",
                     help="config 2: precision of the batched solve (the headline is mixed)")
-    ap.add_argument("--inflight", type=int, default=2,
-                    help="config 2: solves in flight on separate streams (double buffering: the "
-                         "duplicate re-track and tail of one solve overlap the next)")
+    ap.add_argument("--inflight", type=int, default=3,
+                    help="configs 1, 2: solves in flight on separate streams (the duplicate re-track "
+                         "and tail of one solve overlap the next ones)")
+    ap.add_argument("--waves-per-cu", type=int, default=7,
+                    help="config 2: persistent grid density (waves per CU); one below the n = 6 "
+                         "kernel's occupancy leaves slots for the previous solve's re-track and tail")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="end with one 256 MiB device copy, the reference dispatch for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scale factors")
@@ -135,7 +138,7 @@ def main():
     K = prob.num_tracks
     P = I * K
     opts_mixed = hcx.default_opts(precision=hcx.PREC_MIXED if args.precision == "mixed" else hcx.PREC_FP64,
-                                  lanes_per_path=args.lanes)
+                                  lanes_per_path=args.lanes, waves_per_cu=args.waves_per_cu)
     opts_fp64 = hcx.default_opts(precision=hcx.PREC_FP64, lanes_per_path=args.lanes)
     batch = hcx.Batch(prob, I, target_params=p1)
     res = hcx.Result(P, prob.n, I)
@@ -182,7 +185,8 @@ def main():
     # duplicate re-track): reported beside the headline, not instead of it
     fp64_batched = None
     if args.precision == "mixed":
-        opts_b64 = hcx.default_opts(precision=hcx.PREC_FP64, lanes_per_path=args.lanes)
+        opts_b64 = hcx.default_opts(precision=hcx.PREC_FP64, lanes_per_path=args.lanes,
+                                    waves_per_cu=args.waves_per_cu)
         run_steps(opts_b64, args.warmup)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -285,7 +289,8 @@ def main():
         "config": {"workload": "n=6 parameter homotopy, 12 monomials/eq, k=24 params, 64 start roots, "
                                f"{I} target instances ({P} paths per step)",
                    "instances": I, "paths_per_instance": K, "seed": args.seed,
-                   "lanes_per_path": args.lanes or prob.n, "solves_in_flight": nf},
+                   "lanes_per_path": args.lanes or prob.n, "solves_in_flight": nf,
+                   "waves_per_cu": args.waves_per_cu},
         "p50_ms": percentile(lat_ms, 50),
         "p99_ms": percentile(lat_ms, 99),
         "latency_note": "per-instance completion time from batch launch (device clock)",
