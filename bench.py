@@ -144,7 +144,7 @@ def main():
     res = hcx.Result(P, prob.n, I)
     stream = torch.cuda.current_stream()
 
-    # double buffering: each solve in flight has its own problem handle (launch
+    # solves in flight: each has its own problem handle (launch
     # workspace), batch (coefficient workspace), result and stream; the host
     # enqueues solve i before it waits for the counts of solve i - 1
     nf = max(1, args.inflight)
