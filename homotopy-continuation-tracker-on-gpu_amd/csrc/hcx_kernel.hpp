@@ -429,13 +429,18 @@ __device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, cons
 // predictor (Jacobian, dH/dt and LU of the four stages), FP64 Newton
 // corrector (FP32 before the end zone when o.newton32), FP64 refinement; a
 // path that ends without a refined root is re-tracked from t = 0 entirely in
-// FP64 by the same group (escalation).
+// FP64 (escalation): by the same group (KESC), or, for the size-specialised
+// instances n = 9..16, by the FP64 pass after the launch, which keeps the FP64
+// predictor out of the kernel: its register footprint then allows one more
+// wave per SIMD (n = 12: 260 -> 241 VGPRs).  The pass skips paths already
+// escalated, so either way each failed path is re-tracked once.
 template <int NMAX, int DEG, int MODE, bool EXACT>
 __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned char* smem, int lane) {
   // local copies: the fields are wave-uniform and live in SGPRs (scalar
   // kernarg loads) instead of being re-read through a generic pointer
   Seg S = LN.seg[si];
   constexpr int GE = exact_lanes(NMAX);
+  constexpr bool KESC = !(EXACT && NMAX >= 9 && NMAX <= 16);
   if (EXACT) {
     // size-specialised instance: n = NMAX and G are compile-time constants,
     // so every loop bound and the lane-group geometry fold away
@@ -471,7 +476,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
   int inst = 0, root = 0;
   double t = 0.0, dt = 0.0;
   int cnt = 0, nsteps = 0, asteps = 0, nnewt = 0, nref = 0;
-  bool ez = false, esc = (MODE == 0);
+  bool ez = false, esc = (MODE == 0);  // this path runs in FP64
   int status = 0;  // 0 idle, 1 tracking, 2 finishing
   uint32_t fl = 0;
   bool exhausted = !g.in_grp;
@@ -557,7 +562,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
       }
       const double ts = t;
       cx<double> acc;
-      if (MODE == 0) {
+      if constexpr (MODE == 0) {
         // dc/dt (constant when M = 1) from the instance's table in global
         // memory halves the group's LDS and raises occupancy; a launch of few
         // paths (latency bound) stages it in LDS instead
@@ -565,6 +570,8 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
                                        ? reinterpret_cast<const cx<double>*>(S.cht + (size_t)inst * (S.C + 1))
                                        : nullptr;
         acc = predict<NMAX, GC, DEG, double, LT>(S, g, L, lt, stepping, row, gle, inst, ts, dt, xl, sd_ext);
+      } else if constexpr (!KESC) {
+        acc = predict<NMAX, GC, DEG, float, LT>(S, g, L, lt, stepping, row, gle, inst, ts, dt, xl, nullptr);
       } else {
         const bool p32 = stepping && !esc, p64 = stepping && esc;
         cx<double> a32 = cmk<double>(0, 0), a64 = a32;
@@ -677,39 +684,37 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
         }
         HCX_WAVE_SYNC();
       }
-      if (fin) {
-        if (MODE == 1 && !esc && !(fl & HCX_FLAG_REFINED)) {
-          // precision escalation: the mixed attempt did not reach a refined
-          // root; track the same path again from t = 0 in FP64
-          fl = HCX_FLAG_ESCALATED;
-          start_path(true);
-        } else {
-          const SegOut* O = S.out;
-          double2* roots = O->roots;
-          const int rstride = O->roots_stride;
-          if (row) roots[(size_t)path * rstride + g.gl] = make_double2(x.re, x.im);
-          if (g.gl == 0) {
-            if (rstride > n) roots[(size_t)path * rstride + n] = make_double2(1.0, 0.0);
-            O->flags[path] = fl;
-            if (O->steps) O->steps[path] = nsteps;
-            if (O->newton) O->newton[path] = nnewt;
-            if (O->refine) O->refine[path] = nref;
-            if (O->tfin) O->tfin[path] = t;
-            if (O->dtfin) O->dtfin[path] = dt;
-            if (O->resid) O->resid[path] = resid;
-            if (O->conv8) O->conv8[path] = (fl & HCX_FLAG_TRACKED) ? 1 : 0;
-            if (O->inf8) O->inf8[path] = (fl & HCX_FLAG_INFINITY) ? 1 : 0;
-            if (O->debug) O->debug[path] = (fl & HCX_FLAG_TRACKED) ? make_double2(1.0, 0.0) : make_double2(t, dt);
-            if (O->inst_left) {
-              const int left = atomicSub(&O->inst_left[inst], 1);
-              if (left == 1 && O->inst_done) O->inst_done[inst] = wall_clock64();
-            } else if (O->inst_done && S.plist) {
-              atomicMax(&O->inst_done[inst], (unsigned long long)wall_clock64());
-            }
+      if (MODE == 1 && KESC && fin && !esc && !(fl & HCX_FLAG_REFINED)) {
+        // precision escalation: the mixed attempt did not reach a refined
+        // root; track the same path again from t = 0 in FP64
+        fl = HCX_FLAG_ESCALATED;
+        start_path(true);
+      } else if (fin) {
+        const SegOut* O = S.out;
+        double2* roots = O->roots;
+        const int rstride = O->roots_stride;
+        if (row) roots[(size_t)path * rstride + g.gl] = make_double2(x.re, x.im);
+        if (g.gl == 0) {
+          if (rstride > n) roots[(size_t)path * rstride + n] = make_double2(1.0, 0.0);
+          O->flags[path] = fl;
+          if (O->steps) O->steps[path] = nsteps;
+          if (O->newton) O->newton[path] = nnewt;
+          if (O->refine) O->refine[path] = nref;
+          if (O->tfin) O->tfin[path] = t;
+          if (O->dtfin) O->dtfin[path] = dt;
+          if (O->resid) O->resid[path] = resid;
+          if (O->conv8) O->conv8[path] = (fl & HCX_FLAG_TRACKED) ? 1 : 0;
+          if (O->inf8) O->inf8[path] = (fl & HCX_FLAG_INFINITY) ? 1 : 0;
+          if (O->debug) O->debug[path] = (fl & HCX_FLAG_TRACKED) ? make_double2(1.0, 0.0) : make_double2(t, dt);
+          if (O->inst_left) {
+            const int left = atomicSub(&O->inst_left[inst], 1);
+            if (left == 1 && O->inst_done) O->inst_done[inst] = wall_clock64();
+          } else if (O->inst_done && S.plist) {
+            atomicMax(&O->inst_done[inst], (unsigned long long)wall_clock64());
           }
-          status = 0;
-          path = -1;
         }
+        status = 0;
+        path = -1;
       }
     }
   }

@@ -196,16 +196,28 @@ __global__ void collect_kernel(int n, int K, const double2* __restrict__ roots, 
   atomicAdd(&counts[4], c_uni);
 }
 
-// Marks every refined path whose root equals another refined root of the
-// same instance (|dRe|, |dIm| < tol in every coordinate): a path that jumped
-// onto another path's root shares that root.  The marked paths are appended
-// to plist for an FP64 re-track.  One block per instance.
+// Lists the paths of a mixed-precision launch that the FP64 pass re-tracks
+// from t = 0: every path without a refined root that the launch did not
+// already re-track in FP64 (precision escalation) and, when mark_dups is set,
+// every refined path whose root equals another refined root of the same
+// instance (|dRe|, |dIm| < tol in every coordinate): a path that jumped onto
+// another path's root shares that root.  The listed paths are appended to
+// plist.  One block per instance.
 __global__ void dup_mark_kernel(int n, int K, const double2* __restrict__ roots, const uint32_t* __restrict__ flags,
-                                double tol, int* __restrict__ plist, unsigned int* __restrict__ count) {
+                                double tol, int mark_dups, int* __restrict__ plist,
+                                unsigned int* __restrict__ count) {
   const int inst = blockIdx.x;
   const size_t base = (size_t)inst * K;
   for (int j = threadIdx.x; j < K; j += blockDim.x) {
-    if (!(flags[base + j] & HCX_FLAG_REFINED)) continue;
+    const uint32_t fj = flags[base + j];
+    if (!(fj & HCX_FLAG_REFINED)) {
+      if (!(fj & HCX_FLAG_ESCALATED)) {
+        const unsigned int q = atomicAdd(count, 1u);
+        plist[q] = (int)(base + j);
+      }
+      continue;
+    }
+    if (!mark_dups) continue;
     const double2* xj = roots + (base + j) * n;
     bool dup = false;
     for (int i = 0; i < K && !dup; ++i) {
@@ -969,11 +981,13 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
   hipLaunchKernelGGL(pl.fn, dim3((unsigned)blocks), dim3(64), pl.lds, stream, LN);
   HIP_TRY(hipGetLastError());
 
-  // Mixed precision: re-track in FP64 every path whose root duplicates another
-  // root of its instance (a jump onto another path), so the refined roots and
-  // counts are those of the FP64 tracker.
-  if (persistent && o->precision == HCX_PREC_MIXED && o->retrack_duplicates) {
+  // Mixed precision: re-track in FP64 from t = 0 every path that ended
+  // without a refined root (precision escalation) and every path whose root
+  // duplicates another root of its instance (a jump onto another path), so
+  // the refined roots and counts are those of the FP64 tracker.
+  if (o->precision == HCX_PREC_MIXED) {
     Launch L2 = LN;
+    L2.persistent = 1;
     OutPack OP2 = OP;
     for (int s = 0; s < nseg; ++s) {
       hcx_problem* p = probs[s];
@@ -997,7 +1011,7 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
       if (S.I > 0)
         hipLaunchKernelGGL(dup_mark_kernel, dim3(S.I), dim3(256), 0, stream, S.n, S.K,
                            reinterpret_cast<const double2*>(results[s].d_roots), results[s].d_flags, o->unique_tol,
-                           plist, pcount);
+                           o->retrack_duplicates, plist, pcount);
       S.plist = plist;
       S.plist_count = pcount;
       OP2.out[s].inst_left = nullptr;

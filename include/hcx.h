@@ -54,7 +54,8 @@ int hcx_device_count(void);
 typedef enum {
   HCX_PREC_FP64 = 0,   /* everything complex FP64 */
   HCX_PREC_MIXED = 1   /* FP32 RK4 predictor (Jacobian, dH/dt, LU); FP64 Newton corrector (FP32 before the
-                          end zone when fp32_corrector = 1); FP64 refinement */
+                          end zone when fp32_corrector = 1); FP64 refinement; a path that ends without a
+                          refined root is re-tracked from t = 0 in FP64 after the launch (escalation) */
 } hcx_precision;
 
 typedef struct {
