@@ -190,7 +190,9 @@ hcx_status hcx_track(hcx_problem* p, const hcx_batch_desc* b, const hcx_track_op
 /* Same work with the reference's launch shape: each lane group tracks exactly
  * one path and exits (grid = paths / groups per wave, no refill, no
  * persistence) — one launch per problem instance is the plain per-problem
- * baseline (kernel_HC_Solver_generalized_3views_4pts.cu:299-300,361-371). */
+ * baseline (kernel_HC_Solver_generalized_3views_4pts.cu:299-300,361-371).
+ * In mixed precision the FP64 re-track pass (escalations, duplicate roots)
+ * follows as after hcx_track. */
 hcx_status hcx_track_static(hcx_problem* p, const hcx_batch_desc* b, const hcx_track_opts* o,
                             const hcx_result_desc* r, void* stream);
 
