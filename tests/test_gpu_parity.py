@@ -100,12 +100,16 @@ def test_config2_parameter_homotopy(hcx_mod, oracle_mod, config2_small, prec):
     assert gc["refined"] == oc["refined"] and gc["real"] == oc["real"] and gc["unique"] == oc["unique"]
 
 
-def test_static_launch_equals_persistent(hcx_mod, config2_small):
+@pytest.mark.parametrize("prec", [0, 1])
+def test_static_launch_equals_persistent(hcx_mod, config2_small, prec):
+    """The reference's launch shape (one path per group, no refill) gives the
+    persistent launch's results; in mixed precision both are followed by the
+    FP64 pass (escalations, duplicate roots)."""
     import torch
     hcx = hcx_mod
     sysf, p0, start, p1 = config2_small
     prob = hcx.Problem.from_system(sysf, start, p0=p0)
-    opts = hcx.default_opts()
+    opts = hcx.default_opts(precision=prec)
     b = hcx.Batch(prob, 4, target_params=p1[:4])
     a = hcx.track(prob, b, opts).to_host()
     s = hcx.track(prob, b, opts, static=True).to_host()
