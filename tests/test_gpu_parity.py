@@ -176,6 +176,34 @@ def test_multi_segment_launch(hcx_mod, oracle_mod, gpu):
         assert (mh["inst_done"][: b.I] > 0).all()
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_multi_segment_same_size(hcx_mod, gpu, prec):
+    """Three different n = 6 problems in one launch run on the size-specialised
+    instance, each wave copying the next segment's compact tables into LDS when
+    it moves on; every segment equals its separate launch bit for bit."""
+    import torch
+    from hcx import workloads
+    hcx = hcx_mod
+    probs, batches = [], []
+    for seed in (41, 42, 43):
+        sysf = workloads.quadratic_type(6, 12, seed)
+        p0 = hcx.random_cn(seed + 100, 12)
+        start = workloads.seed_start_roots(sysf, p0, 64, seed=seed)
+        prob = hcx.Problem.from_system(sysf, start, p0=p0)
+        probs.append(prob)
+        batches.append(hcx.Batch(prob, 6, target_params=workloads.targets(12, 6, seed + 200)))
+    opts = hcx.default_opts(precision=prec)
+    multi = hcx.track_multi(probs, batches, opts)
+    torch.cuda.synchronize()
+    for prob, b, m in zip(probs, batches, multi):
+        s = hcx.track(prob, b, opts).to_host()
+        mh = m.to_host()
+        torch.cuda.synchronize()
+        assert (mh["flags"] == s["flags"]).all()
+        assert (mh["steps"] == s["steps"]).all()
+        assert (mh["roots"] == s["roots"]).all()
+
+
 def test_reference_shaped_launcher(hcx_mod, oracle_mod, gpu):
     """hcx_kernel_HC_Solver: [K][n+1] buffers, bool flags, (t, dt) debug, elapsed seconds."""
     import torch
