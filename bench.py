@@ -45,17 +45,83 @@ def profiled_traffic():
         return None
 
 
+def flops_split(info, h, precision_mixed):
+    """Algorithmic FLOPs (LAWN 41) of a solve's paths from the per-path device
+    counters, split by the precision they ran in: in mixed precision the RK4
+    stages of a path run in FP32 unless the path was re-tracked in FP64
+    (HCX_FLAG_ESCALATED), and the corrector, refinement and coefficient
+    evaluation at t = 1 run in FP64."""
+    import numpy as np
+    import hcx
+    steps = h["steps"].astype(np.float64)
+    newton = h["newton"].astype(np.float64)
+    refine = h["refine"].astype(np.float64)
+    stage = 4.0 * steps * info["flops_stage"]
+    rest = (newton + refine) * info["flops_newton"] + (refine > 0) * info["flops_coef"]
+    if not precision_mixed:
+        return 0.0, float((stage + rest).sum())
+    esc = (h["flags"] & hcx.FLAG_ESCALATED) != 0
+    f32 = float(stage[~esc].sum())
+    f64 = float(stage[esc].sum() + rest.sum())
+    return f32, f64
+
+
+def roofline_valu(f32, f64, seconds, kernel_ms=None):
+    """Vector-FLOP roofline of the solve: achieved TFLOP/s over the timed
+    steady state against MI355X's published vector peaks for the same
+    FP32/FP64 instruction mix (157.3 FP32, 78.6 FP64 TFLOP/s)."""
+    total = f32 + f64
+    achieved = total / seconds / 1e12
+    peak = total / (f32 / PEAK_FP32_PACKED + f64 / PEAK_VALU_TFLOPS) if total > 0 else PEAK_VALU_TFLOPS
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "algorithmic_flop": total, "flop_fp32": f32, "flop_fp64": f64,
+            "achieved_fp32": f32 / seconds / 1e12, "achieved_fp64": f64 / seconds / 1e12,
+            "frac_vs_unpacked_peak": achieved / PEAK_VALU_TFLOPS,
+            "kernel_ms": kernel_ms,
+            "note": "LAWN-41 FLOPs of the reference algorithm from per-path device step/Newton counters over the "
+                    "timed steady state (solves in flight included); peak = published vector peaks weighted by "
+                    "the FP32/FP64 mix; FP32 runs unpacked here (78.6 TFLOP/s issue bound, frac_vs_unpacked_peak)"}
+
+
 def percentile(a, q):
     import numpy as np
     return float(np.percentile(np.asarray(a, dtype=float), q)) if len(a) else float("nan")
 
 
-def cpu_baseline(sysf, p0, start, p1, seconds):
-    """Oracle (FP64 C++ restatement) on host cores over a bounded sample."""
+def host_threads():
+    """Threads the CPU baseline may use: the cores this process may run on
+    (sched_getaffinity), at most 16 -- the host share of one GPU on the
+    benchmark machines, whatever the machine reports in total."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail)), avail, os.cpu_count() or 1
+
+
+def _cpu_rate(op, start, hx, ht, threads, seconds):
     import numpy as np
     import oracle
-    # the host share of one GPU is 16 cores, whatever the machine reports
-    threads = min(16, os.cpu_count() or 1)
+    conv = paths = reps = 0
+    n_inst = hx.shape[0]
+    t0 = time.perf_counter()
+    while True:
+        out = op.track(start, hx, ht, threads=threads)
+        conv += int(((out["flags"] & oracle.FLAG_REFINED) != 0).sum())
+        paths += n_inst * len(start)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            return conv / el, paths / el, reps, paths, el
+
+
+def cpu_baseline(sysf, p0, start, p1, seconds):
+    """Oracle (FP64 C++ restatement of the reference's tracker) on host cores
+    over a bounded sample of the same workload: all usable threads (the
+    headline value) and one thread."""
+    import numpy as np
+    import oracle
+    threads, avail, total = host_threads()
     hxi, hti = sysf.tables()
     op = oracle.OracleProblem(sysf.info["num_vars"], sysf.info["num_coeffs"], hxi, hti, 1)
     A = sysf.affine()
@@ -67,20 +133,18 @@ def cpu_baseline(sysf, p0, start, p1, seconds):
     n_inst = int(max(1, min(len(p1), seconds / max(t1, 1e-4))))
     hx, ht = oracle.p2c_affine(A, p0, p1[:n_inst])
     # repeat the sample until it holds about `seconds` of CPU work
-    conv = paths = reps = 0
-    t0 = time.perf_counter()
-    while True:
-        out = op.track(start, hx, ht, threads=threads)
-        conv += int(((out["flags"] & oracle.FLAG_REFINED) != 0).sum())
-        paths += n_inst * len(start)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    return {"value": conv / el, "unit": "converged paths/s", "cores": threads, "kind": "port",
+    value, pps, reps, paths, el = _cpu_rate(op, start, hx, ht, threads, seconds)
+    # one thread on a smaller sample of the same instances
+    n1 = max(1, n_inst // max(1, threads))
+    v1, pps1, reps1, paths1, el1 = _cpu_rate(op, start, hx[:n1], ht[:n1], 1, seconds * 0.5)
+    return {"value": value, "unit": "converged paths/s", "cores": threads, "kind": "port",
             "sample": f"{n_inst} instances x {len(start)} paths of the same workload, tracked {reps} times "
                       f"({paths} paths), FP64 oracle, {threads} threads, {el:.1f} s",
-            "paths_per_s": paths / el}
+            "paths_per_s": pps,
+            "single_thread": {"value": v1, "unit": "converged paths/s", "cores": 1,
+                              "sample": f"{n1} instances x {len(start)} paths, tracked {reps1} times "
+                                        f"({paths1} paths), {el1:.1f} s"},
+            "host_cpus": {"usable": avail, "reported": total, "used": threads}}
 
 
 def main():
@@ -102,6 +166,11 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=7,
                     help="config 2: persistent grid density (waves per CU); one below the n = 6 "
                          "kernel's occupancy leaves slots for the previous solve's re-track and tail")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="default run: seconds of each serving window (fixed rate, then the offered 2000 req/s)")
+    ap.add_argument("--serve-rate", type=float, default=800.0,
+                    help="default run: the fixed Poisson request rate the latency is stated at")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="end with one 256 MiB device copy, the reference dispatch for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scale factors")
@@ -144,14 +213,14 @@ def main():
     res = hcx.Result(P, prob.n, I)
     stream = torch.cuda.current_stream()
 
-    # solves in flight: each has its own problem handle (launch
-    # workspace), batch (coefficient workspace), result and stream; the host
-    # enqueues solve i before it waits for the counts of solve i - 1
+    # solves in flight: one resident problem; each solve has its own batch
+    # (coefficient workspace), result and stream (launch workspaces belong to
+    # streams); the host enqueues solve i before it waits for the counts of
+    # solve i - 1
     nf = max(1, args.inflight)
     lanes = [(prob, batch, res, stream)]
     for _ in range(nf - 1):
-        pk = hcx.Problem.from_system(sysf, start, p0=p0)
-        lanes.append((pk, hcx.Batch(pk, I, target_params=p1), hcx.Result(P, prob.n, I), torch.cuda.Stream()))
+        lanes.append((prob, hcx.Batch(prob, I, target_params=p1), hcx.Result(P, prob.n, I), torch.cuda.Stream()))
 
     def run_steps(opts, count):
         counts = None
@@ -179,6 +248,7 @@ def main():
     value = counts["refined"] / (elapsed / args.steps)
     h = res.to_host()
     lat_ms = (h["inst_done"][:I] - h["batch_start"][0]) / DEVICE_CLOCK_HZ * 1e3
+    f32, f64 = flops_split(info, h, args.precision == "mixed")
     log(f"bench: mixed {ms_per_step:.2f} ms/step, {value:.4g} converged paths/s, counts {counts}")
 
     # the same batched loop with the FP64 tracker (no FP32 stages, no
@@ -212,10 +282,11 @@ def main():
     ev[1].record(stream)
     torch.cuda.synchronize()
     kernel_ms = ev[0].elapsed_time(ev[1]) / reps
-    kh = kres.to_host()
-    flops = float(hcx.flops_per_path(info, kh["steps"], kh["newton"], kh["refine"]).sum())
-    achieved = flops / (kernel_ms * 1e-3) / 1e12
-    log(f"bench: track kernel {kernel_ms:.3f} ms, {flops:.4g} algorithmic FLOP -> {achieved:.2f} TFLOP/s")
+    roof = roofline_valu(f32, f64, ms_per_step * 1e-3, kernel_ms=kernel_ms)
+    roof["traffic"] = profiled_traffic()
+    roof["traffic_unit"] = "bytes/launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)"
+    log(f"bench: one solve alone {kernel_ms:.3f} ms; steady state {roof['algorithmic_flop']:.4g} FLOP per step "
+        f"(FP32 {f32:.3g}, FP64 {f64:.3g}) -> {roof['achieved']:.2f} TFLOP/s, frac {roof['frac']:.4f}")
 
     # ------------------------------------------------------------ plain FP64 per-problem
     fp64 = None
@@ -272,6 +343,10 @@ def main():
         cpu = cpu_baseline(sysf, p0, start, p1, args.cpu_seconds)
         log(f"bench: cpu {cpu}")
 
+    serving = None
+    if not args.no_serve:
+        serving = serving_leg(args)
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -291,20 +366,19 @@ def main():
                    "instances": I, "paths_per_instance": K, "seed": args.seed,
                    "lanes_per_path": args.lanes or prob.n, "solves_in_flight": nf,
                    "waves_per_cu": args.waves_per_cu},
-        "p50_ms": percentile(lat_ms, 50),
-        "p99_ms": percentile(lat_ms, 99),
-        "latency_note": "per-instance completion time from batch launch (device clock)",
+        "p50_ms": serving["fixed_rate"]["p50_ms"] if serving else None,
+        "p99_ms": serving["fixed_rate"]["p99_ms"] if serving else None,
+        "latency_note": (f"solve latency (submit -> result on the host) of the serving loop at a fixed "
+                         f"{args.serve_rate:.0f} req/s Poisson stream over the six problem types; see serving"),
+        "serving": serving,
+        "batch_instance_completion_ms": {"p50": percentile(lat_ms, 50), "p99": percentile(lat_ms, 99),
+                                         "note": "per-instance completion time from batch launch (device clock)"},
         "counts": counts,
         "fp64_per_problem": fp64,
         "speedup_vs_fp64_per_problem": (value / fp64["value"]) if fp64 else None,
         "fp64_batched": fp64_batched,
         "parity_mixed_vs_fp64": parity,
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_VALU_TFLOPS, "traffic": profiled_traffic(),
-                     "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
-                     "kernel_ms": kernel_ms, "algorithmic_flop": flops,
-                     "note": "LAWN-41 FLOPs of the reference algorithm from per-path device step/Newton "
-                             "counters; peak = vector FP64 (= non-packed FP32 rate)"},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -368,13 +442,12 @@ def bench_config1(args):
         hxs.append(hx)
         hts.append(ht)
     hxs, hts = np.stack(hxs), np.stack(hts)
-    # solves in flight on separate streams, each with its own problem handle
-    # (launch workspace), batch and result
+    # solves in flight on separate streams (one resident problem; each solve
+    # has its own batch and result, launch workspaces belong to the streams)
     nf = max(1, args.inflight)
     lanes = [(prob, hcx.Batch(prob, R, coeffs_hx=hxs, coeffs_ht=hts), hcx.Result(R * 8, 3, R), stream)]
     for _ in range(nf - 1):
-        pk = hcx.Problem(3, nc, hxi, hti, c["start"])
-        lanes.append((pk, hcx.Batch(pk, R, coeffs_hx=hxs, coeffs_ht=hts), hcx.Result(R * 8, 3, R),
+        lanes.append((prob, hcx.Batch(prob, R, coeffs_hx=hxs, coeffs_ht=hts), hcx.Result(R * 8, 3, R),
                       torch.cuda.Stream()))
     opts = hcx.default_opts(precision=hcx.PREC_MIXED)
     for pk, bk, rk, sk in lanes:
@@ -568,6 +641,36 @@ def _serve(srv, tids, types, rate, duration, seed):
             "requests_per_s": st["completed"] / t_end, "converged_paths_per_s": st["converged_paths"] / t_end,
             "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "max_ms": st["max_ms"],
             "submit_seconds": t_sub, "total_seconds": t_end, "max_submit_lateness_ms": late * 1e3, "stats": st}
+
+
+def serving_leg(args):
+    """Bounded serving windows for the default run: Poisson arrivals over the
+    six resident types n in {3, 6, 9, 12, 16, 30} at a fixed rate (the stated
+    p50 / p99 solve latency) and at the offered 2000 req/s."""
+    import hcx
+    from hcx import workloads
+    log("bench: serving leg, seeding start roots of six types")
+    types = workloads.quadratic_types(workloads.CONFIG5_TYPES, seed=args.seed)
+    probs = [hcx.Problem.from_system(t["system"], t["start"], p0=t["p0"]) for t in types]
+    srv = hcx.Server(hcx.default_opts(precision=hcx.PREC_MIXED), hbm_budget_bytes=4 << 30, max_batch_requests=64,
+                     streams=4)
+    tids = [srv.add_type(p) for p in probs]
+    for ti, ty in enumerate(types):
+        srv.submit(tids[ti], workloads.targets(ty["k"], 1, 90000 + ti)[0])
+    srv.drain()
+    out = {"types": [[t["n"], t["K"]] for t in types], "hbm_budget_bytes": 4 << 30, "streams": 4,
+           "max_batch_requests": 64}
+    for key, rate, seed in (("fixed_rate", args.serve_rate, args.seed + 1), ("offered_2000", 2000.0, args.seed)):
+        r = _serve(srv, tids, types, rate, args.serve_seconds, seed=seed)
+        out[key] = {"offered_rps": rate, "served_rps": r["requests_per_s"], "p50_ms": r["p50_ms"],
+                    "p99_ms": r["p99_ms"], "mean_ms": r["mean_ms"], "max_ms": r["max_ms"],
+                    "converged_paths_per_s": r["converged_paths_per_s"], "requests": r["requests"],
+                    "seconds": r["total_seconds"]}
+        log(f"bench: serving {key}: {out[key]}")
+    srv.close()
+    for p in probs:
+        p.close()
+    return out
 
 
 def _fp64_per_problem_mix(probs, types, per_type, seed):

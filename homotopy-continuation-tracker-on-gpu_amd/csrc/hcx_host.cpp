@@ -169,6 +169,7 @@ void hcx_track_opts_default(hcx_track_opts* o) {
   o->specialize = 1;
   o->retrack_duplicates = 1;
   o->fp32_corrector = 0;
+  o->real_ref_tol = 1e-4;
 }
 
 hcx_status hcx_random_cn(uint64_t seed, int64_t count, hcx_complex* out) {

@@ -1,0 +1,8 @@
+// Generic tracker kernel instance: NMAX = 16, terms of up to 6 variable factors, FP64 mode.
+#include "hcx_kernel.hpp"
+
+namespace hcx {
+
+KernelFn kernel_g16_m0_d6() { return track_kernel<16, 6, 0, false>; }
+
+}  // namespace hcx

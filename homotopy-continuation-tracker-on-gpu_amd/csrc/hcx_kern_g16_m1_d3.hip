@@ -1,0 +1,8 @@
+// Generic tracker kernel instance: NMAX = 16, terms of up to 3 variable factors, mixed FP32/FP64 mode.
+#include "hcx_kernel.hpp"
+
+namespace hcx {
+
+KernelFn kernel_g16_m1_d3() { return track_kernel<16, 3, 1, false>; }
+
+}  // namespace hcx

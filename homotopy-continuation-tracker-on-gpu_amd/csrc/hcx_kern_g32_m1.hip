@@ -1,13 +1,15 @@
 // Generic tracker kernel instances: NMAX = 32 (any n <= 32, any G >= n), mixed FP32/FP64 mode.
+// One translation unit per term degree (hcx_kern_g32_m1_d*.hip) so that
+// the instances compile in parallel; this unit only dispatches.
 #include "hcx_kernel.hpp"
 
 namespace hcx {
 
 KernelFn kernel_g32_m1(int deg) {
   switch (deg) {
-    case 2: return track_kernel<32, 2, 1, false>;
-    case 3: return track_kernel<32, 3, 1, false>;
-    case 6: return track_kernel<32, 6, 1, false>;
+    case 2: return kernel_g32_m1_d2();
+    case 3: return kernel_g32_m1_d3();
+    case 6: return kernel_g32_m1_d6();
     default: return nullptr;
   }
 }

@@ -1,13 +1,15 @@
 // Generic tracker kernel instances: NMAX = 8 (any n <= 8, any G >= n), FP64 mode.
+// One translation unit per term degree (hcx_kern_g8_m0_d*.hip) so that
+// the instances compile in parallel; this unit only dispatches.
 #include "hcx_kernel.hpp"
 
 namespace hcx {
 
 KernelFn kernel_g8_m0(int deg) {
   switch (deg) {
-    case 2: return track_kernel<8, 2, 0, false>;
-    case 3: return track_kernel<8, 3, 0, false>;
-    case 6: return track_kernel<8, 6, 0, false>;
+    case 2: return kernel_g8_m0_d2();
+    case 3: return kernel_g8_m0_d3();
+    case 6: return kernel_g8_m0_d6();
     default: return nullptr;
   }
 }

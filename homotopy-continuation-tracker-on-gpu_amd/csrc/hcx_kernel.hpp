@@ -42,13 +42,14 @@ struct SegOut {
 };
 
 // Per-launch device workspace of one segment (path queue, output
-// descriptor, per-instance remaining-path counters).
+// descriptor, per-instance remaining-path counters, re-track path list).
 struct LaunchWork {
   unsigned int* counter;
   SegOut* out;
-  int* inst_left;
+  int* inst_left;              // capacity inst_cap
   int* plist;                  // re-track path list (capacity plist_cap)
   unsigned int* plist_count;   // [1]
+  long long inst_cap;
   long long plist_cap;
 };
 
@@ -80,6 +81,7 @@ struct Opts {
   int max_steps, max_corr, incr, refine_iters;
   int newton32;      // mixed: FP32 corrector before the end zone
   double dt0, t_tol, dt_min, corr_tol, inf_tol, refine_tol, real_tol;
+  double real_ref_tol;  // the reference's real test on the tracked root
   double ez_enter;   // enter the end zone when |1 - t| <= end_zone + 1e-7
   double ez_land;    // before the end zone, steps land on 1 - end_zone
 };
@@ -636,6 +638,14 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
     // ------------------------------------------------ finish: refine + outputs
     const bool fin = status == 2;
     if (__any(fin)) {
+      // the reference's real test on the tracked root, before refinement
+      // (Evaluations.cpp:96-104: converged and every |Im x_i| <= 1e-4)
+      {
+        double imt = (fin && row) ? fabs(x.im) : 0.0;
+        if (imt != imt) imt = INFINITY;
+        imt = group_max(imt, g);
+        if (fin && (fl & HCX_FLAG_TRACKED) && imt <= o.real_ref_tol) fl |= HCX_FLAG_REAL_REF;
+      }
       const bool ref = fin && (fl & HCX_FLAG_TRACKED) && o.refine_iters > 0;
       double resid = 0.0;
       if (__any(ref)) {
@@ -686,8 +696,12 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
       }
       if (MODE == 1 && KESC && fin && !esc && !(fl & HCX_FLAG_REFINED)) {
         // precision escalation: the mixed attempt did not reach a refined
-        // root; track the same path again from t = 0 in FP64
+        // root; track the same path again from t = 0 in FP64 (the counters
+        // then describe the FP64 attempt, as after the re-track pass)
         fl = HCX_FLAG_ESCALATED;
+        nsteps = 0;
+        nnewt = 0;
+        nref = 0;
         start_path(true);
       } else if (fin) {
         const SegOut* O = S.out;
@@ -696,7 +710,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
         if (row) roots[(size_t)path * rstride + g.gl] = make_double2(x.re, x.im);
         if (g.gl == 0) {
           if (rstride > n) roots[(size_t)path * rstride + n] = make_double2(1.0, 0.0);
-          O->flags[path] = fl;
+          if (O->flags) O->flags[path] = fl;
           if (O->steps) O->steps[path] = nsteps;
           if (O->newton) O->newton[path] = nnewt;
           if (O->refine) O->refine[path] = nref;
@@ -736,13 +750,23 @@ __global__ __launch_bounds__(64) void track_kernel(const Launch LN) {
 using KernelFn = void (*)(const Launch);
 
 // Kernel instances live in separate translation units (compile in parallel).
-// Generic instances: any n <= NMAX and any lanes-per-path G >= n.
+// Generic instances: any n <= NMAX and any lanes-per-path G >= n; one unit
+// per (NMAX, mode, term degree) and a dispatcher per (NMAX, mode).
 KernelFn kernel_g8_m0(int deg);
 KernelFn kernel_g8_m1(int deg);
 KernelFn kernel_g16_m0(int deg);
 KernelFn kernel_g16_m1(int deg);
 KernelFn kernel_g32_m0(int deg);
 KernelFn kernel_g32_m1(int deg);
+#define HCX_GENERIC_UNITS(G, M) \
+  KernelFn kernel_g##G##_m##M##_d2(); KernelFn kernel_g##G##_m##M##_d3(); KernelFn kernel_g##G##_m##M##_d6();
+HCX_GENERIC_UNITS(8, 0)
+HCX_GENERIC_UNITS(8, 1)
+HCX_GENERIC_UNITS(16, 0)
+HCX_GENERIC_UNITS(16, 1)
+HCX_GENERIC_UNITS(32, 0)
+HCX_GENERIC_UNITS(32, 1)
+#undef HCX_GENERIC_UNITS
 // Size-specialised instances (n = G = N) for the problem sizes of the
 // benchmark workloads, DEG 2 and 3.
 KernelFn kernel_x3_m0(int deg);

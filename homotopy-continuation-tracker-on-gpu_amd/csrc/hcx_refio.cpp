@@ -301,11 +301,45 @@ hcx_status hcx_ref_write_converged(const char* path, int n, int num_instances, i
   return HCX_OK;
 }
 
-// Root-level collection.  "Unique" counts the first occurrence of each refined
-// root of an instance: path j is a duplicate when a refined path i < j of the
-// same instance agrees in every coordinate (|dRe|, |dIm| < tol).  The
-// reference's Find_Unique_Sols (Evaluations.cpp:127-176) drops BOTH members of
-// a duplicate pair; this counts one of them.
+// Root-level collection (the host twin of hcx_collect).  "Unique" counts the
+// first occurrence of each refined root of an instance: path j is a duplicate
+// when a refined path i < j of the same instance agrees in every coordinate
+// (|dRe|, |dIm| < tol).  "unique_ref" and "real_ref" are the reference's own
+// counts (Evaluations.cpp:88-110, 127-176): see unique_ref_pass.
+namespace {
+
+bool same_root(const hcx_complex* a, const hcx_complex* b, int n, double tol) {
+  for (int v = 0; v < n; ++v)
+    if (!(std::fabs(a[v].re - b[v].re) < tol && std::fabs(a[v].im - b[v].im) < tol)) return false;
+  return true;
+}
+
+// Find_Unique_Sols (Evaluations.cpp:127-176) over one instance.  Path bs is
+// examined when it is tracked ("converged") and not in the Skip set; its
+// Duplicate set is every LATER path ds (tracked or not) with the same root;
+// an empty set counts bs as unique, a non-empty one REPLACES the Skip set.
+// The Skip set is therefore always the Duplicate set of the last examined
+// path that had one (bstar), and "bs in Skip" is same_root(bs, bstar).
+int64_t unique_ref_pass(int n, int K, const hcx_complex* x, uint32_t* f, double tol) {
+  int64_t cnt = 0;
+  int bstar = -1;
+  for (int bs = 0; bs < K; ++bs) {
+    if (!(f[bs] & HCX_FLAG_TRACKED)) continue;
+    if (bstar >= 0 && same_root(x + (size_t)bs * n, x + (size_t)bstar * n, n, tol)) continue;
+    bool later = false;
+    for (int ds = bs + 1; ds < K && !later; ++ds) later = same_root(x + (size_t)bs * n, x + (size_t)ds * n, n, tol);
+    if (later) {
+      bstar = bs;
+    } else {
+      f[bs] |= HCX_FLAG_UNIQUE_REF;
+      ++cnt;
+    }
+  }
+  return cnt;
+}
+
+}  // namespace
+
 hcx_status hcx_collect_host(int n, int num_instances, int num_tracks, const hcx_complex* h_roots,
                             uint32_t* h_flags, double unique_tol, hcx_counts* out) {
   clear_error();
@@ -318,9 +352,10 @@ hcx_status hcx_collect_host(int n, int num_instances, int num_tracks, const hcx_
     const size_t base = (size_t)inst * num_tracks;
     for (int j = 0; j < num_tracks; ++j) {
       uint32_t& fj = h_flags[base + j];
-      fj &= ~HCX_FLAG_UNIQUE;
+      fj &= ~(HCX_FLAG_UNIQUE | HCX_FLAG_UNIQUE_REF);
       if (fj & HCX_FLAG_TRACKED) out->tracked++;
       if (fj & HCX_FLAG_INFINITY) out->infinity++;
+      if ((fj & HCX_FLAG_TRACKED) && (fj & HCX_FLAG_REAL_REF)) out->real_ref++;
       if (!(fj & HCX_FLAG_REFINED)) continue;
       out->refined++;
       if (fj & HCX_FLAG_REAL) out->real++;
@@ -328,17 +363,14 @@ hcx_status hcx_collect_host(int n, int num_instances, int num_tracks, const hcx_
       bool dup = false;
       for (int i = 0; i < j && !dup; ++i) {
         if (!(h_flags[base + i] & HCX_FLAG_REFINED)) continue;
-        const hcx_complex* xi = h_roots + (base + i) * n;
-        bool same = true;
-        for (int v = 0; v < n && same; ++v)
-          same = std::fabs(xi[v].re - xj[v].re) < unique_tol && std::fabs(xi[v].im - xj[v].im) < unique_tol;
-        dup = same;
+        dup = same_root(h_roots + (base + i) * n, xj, n, unique_tol);
       }
       if (!dup) {
         fj |= HCX_FLAG_UNIQUE;
         out->unique++;
       }
     }
+    out->unique_ref += unique_ref_pass(n, num_tracks, h_roots + base * n, h_flags + base, unique_tol);
   }
   return HCX_OK;
 }

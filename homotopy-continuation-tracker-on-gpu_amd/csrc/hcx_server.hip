@@ -94,14 +94,23 @@ struct Slot {
   hcx_complex* h_roots = nullptr;
   uint32_t* h_flags = nullptr;
   size_t cap_params = 0, cap_roots = 0, cap_paths = 0;
-  // launch workspace (path queue, output descriptor, instance counters)
-  unsigned int* d_counter = nullptr;
-  hcx::SegOut* d_out = nullptr;
-  int* d_inst_left = nullptr;
+  // the slot's launch workspace (path queue, output descriptor, instance
+  // counters, FP64 re-track list): one launch at a time uses it
+  hcx::WsBlock ws;
   unsigned long long* d_inst_done = nullptr;
-  int* d_plist = nullptr;  // FP64 re-track path list (mixed precision)
-  long long plist_cap = 0;
 };
+
+// (Re)allocates a slot's workspace block; only while the slot is idle.
+bool slot_ws_alloc(Slot& sl, long long inst, long long paths) {
+  if (sl.ws.base) (void)hipFree(sl.ws.base);
+  sl.ws = hcx::WsBlock();
+  void* p = nullptr;
+  if (hipMalloc(&p, hcx::ws_layout(sizeof(hcx::SegOut), inst, paths).bytes) != hipSuccess) return false;
+  sl.ws.base = p;
+  sl.ws.inst_cap = inst;
+  sl.ws.paths_cap = paths;
+  return true;
+}
 
 size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 
@@ -298,8 +307,7 @@ struct hcx_server {
     r.d_roots = d_roots;
     r.d_flags = d_flags;
     r.d_inst_done = reinterpret_cast<uint64_t*>(s.d_inst_done);
-    // d_counter holds 64 words: word 0 is the path queue, word 32 the re-track list length
-    hcx::LaunchWork w{s.d_counter, s.d_out, s.d_inst_left, s.d_plist, s.d_counter + 32, s.plist_cap};
+    const hcx::LaunchWork w = hcx::seg_work(s.ws, 0, 0, 0);
     hcx_problem* pp = p;
     const hcx_status ls = hcx::track_impl(1, &pp, &b, &r, &to, s.stream, 1, nullptr, &w);
     if (ls != HCX_OK) {
@@ -384,9 +392,7 @@ hcx_status hcx_server_create(const hcx_server_opts* so, const hcx_track_opts* to
   for (Slot& sl : s->slots) {
     bool ok = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
-              hipMalloc(&sl.d_counter, 64 * sizeof(unsigned int)) == hipSuccess &&
-              hipMalloc(&sl.d_out, sizeof(hcx::SegOut)) == hipSuccess &&
-              hipMalloc(&sl.d_inst_left, sizeof(int) * so->max_batch_requests) == hipSuccess &&
+              slot_ws_alloc(sl, so->max_batch_requests, 0) &&
               hipMalloc(&sl.d_inst_done, sizeof(unsigned long long) * so->max_batch_requests) == hipSuccess;
     if (!ok) {
       hcx::set_error("hcx_server_create: stream/workspace allocation failed");
@@ -418,17 +424,14 @@ hcx_status hcx_server_add_type(hcx_server* s, hcx_problem* p, int* type_id) {
     return HCX_ERR_BUDGET;
   }
   // re-track path lists hold every path of a full batch of the largest type
+  // (no launch is in flight before the first submit)
   const long long need = (long long)s->so.max_batch_requests * p->K;
   for (Slot& sl : s->slots) {
-    if (sl.plist_cap >= need) continue;
-    if (sl.d_plist) (void)hipFree(sl.d_plist);
-    sl.d_plist = nullptr;
-    sl.plist_cap = 0;
-    if (hipMalloc(&sl.d_plist, sizeof(int) * (size_t)need) != hipSuccess) {
-      hcx::set_error("hcx_server_add_type: cannot allocate the re-track path list");
+    if (sl.ws.paths_cap >= need) continue;
+    if (!slot_ws_alloc(sl, s->so.max_batch_requests, need)) {
+      hcx::set_error("hcx_server_add_type: cannot allocate the launch workspace");
       return HCX_ERR_OUT_OF_MEMORY;
     }
-    sl.plist_cap = need;
   }
   s->types.push_back(p);
   s->queues.emplace_back();
@@ -583,10 +586,7 @@ hcx_status hcx_server_destroy(hcx_server* s) {
     if (sl.h_params) (void)hipHostFree(sl.h_params);
     if (sl.h_roots) (void)hipHostFree(sl.h_roots);
     if (sl.h_flags) (void)hipHostFree(sl.h_flags);
-    if (sl.d_counter) (void)hipFree(sl.d_counter);
-    if (sl.d_out) (void)hipFree(sl.d_out);
-    if (sl.d_inst_left) (void)hipFree(sl.d_inst_left);
-    if (sl.d_plist) (void)hipFree(sl.d_plist);
+    if (sl.ws.base) (void)hipFree(sl.ws.base);
     if (sl.d_inst_done) (void)hipFree(sl.d_inst_done);
   }
   if (s->d_arena) (void)hipFree(s->d_arena);

@@ -228,9 +228,11 @@ void GPU_HC_Solver::Solve_by_GPU_HC() {
   check(hcx_collect_host(n, inst_, K, h_roots_.data(), h_flags_.data(), 1e-4, &c), "collect");
   Num_Of_Coverged_Sols = (unsigned)c.tracked;
   Num_Of_Inf_Sols = (unsigned)c.infinity;
+  Num_Of_Real_Sols = (unsigned)c.real_ref;
+  Num_Of_Unique_Sols = (unsigned)c.unique_ref;
   Num_Of_Refined_Sols = (unsigned)c.refined;
-  Num_Of_Real_Sols = (unsigned)c.real;
-  Num_Of_Unique_Sols = (unsigned)c.unique;
+  Num_Of_Refined_Real_Sols = (unsigned)c.real;
+  Num_Of_Distinct_Sols = (unsigned)c.unique;
   if (!opt_.output_dir.empty()) {
     const std::string path = opt_.output_dir + "/GPU_Converged_HC_tracks.txt";
     check(hcx_ref_write_converged(path.c_str(), n, inst_, K, h_roots_.data(), h_flags_.data(), HCX_FLAG_REFINED),
@@ -248,6 +250,8 @@ void GPU_HC_Solver::Solve_by_GPU_HC() {
     std::printf("- infinity failures:            %u\n", Num_Of_Inf_Sols);
     std::printf("- real solutions:               %u\n", Num_Of_Real_Sols);
     std::printf("- unique solutions:             %u\n", Num_Of_Unique_Sols);
+    std::printf("- real refined roots:           %u\n", Num_Of_Refined_Real_Sols);
+    std::printf("- distinct refined roots:       %u\n", Num_Of_Distinct_Sols);
   }
 }
 

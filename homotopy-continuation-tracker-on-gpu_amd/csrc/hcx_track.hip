@@ -158,42 +158,85 @@ __global__ void p2c_poly_kernel(const double2* __restrict__ scal, const int* __r
   }
 }
 
-// Solution collection on the device (Evaluations.cpp:88-176, first-occurrence
-// uniqueness): one block per instance.
+__device__ __forceinline__ bool same_root(const double2* a, const double2* b, int n, double tol) {
+  for (int v = 0; v < n; ++v)
+    if (!(fabs(a[v].x - b[v].x) < tol && fabs(a[v].y - b[v].y) < tol)) return false;
+  return true;
+}
+
+// Solution collection on the device, one block per instance
+// (Evaluations.cpp:88-176).  Counts [tracked, infinity, refined, real,
+// unique, unique_ref, real_ref]:
+//  * unique: first occurrence of a refined root among the refined roots;
+//  * real_ref: tracked paths carrying HCX_FLAG_REAL_REF, the reference's real
+//    test on the tracked root (Evaluate_GPUHC_Sols, :88-110);
+//  * unique_ref: Find_Unique_Sols (:127-176).  Its Skip set is always the
+//    Duplicate set of the last examined path that had one (bstar), so the
+//    sequential pass needs, per path, only "has a later path (any outcome)
+//    with the same root" -- computed in parallel into an LDS bit set -- and a
+//    comparison with bstar.  Dynamic LDS: 3 * ceil(K / 32) words.
 __global__ void collect_kernel(int n, int K, const double2* __restrict__ roots, uint32_t* __restrict__ flags,
                                double tol, unsigned long long* __restrict__ counts) {
+  extern __shared__ unsigned int bits[];
+  const int words = (K + 31) / 32;
+  unsigned int* later = bits;               // a later path has the same root
+  unsigned int* tracked = bits + words;     // tracked ("converged")
+  unsigned int* uref = bits + 2 * words;    // counted by the unique_ref pass
   const int inst = blockIdx.x;
   const size_t base = (size_t)inst * K;
-  unsigned long long c_tr = 0, c_inf = 0, c_ref = 0, c_real = 0, c_uni = 0;
+  for (int w = threadIdx.x; w < 3 * words; w += blockDim.x) bits[w] = 0u;
+  __syncthreads();
+  unsigned long long c_tr = 0, c_inf = 0, c_ref = 0, c_real = 0, c_uni = 0, c_rref = 0;
   for (int j = threadIdx.x; j < K; j += blockDim.x) {
-    uint32_t f = flags[base + j] & ~HCX_FLAG_UNIQUE;
-    c_tr += (f & HCX_FLAG_TRACKED) ? 1 : 0;
+    uint32_t f = flags[base + j] & ~(HCX_FLAG_UNIQUE | HCX_FLAG_UNIQUE_REF);
+    const double2* xj = roots + (base + j) * n;
+    if (f & HCX_FLAG_TRACKED) {
+      ++c_tr;
+      atomicOr(&tracked[j >> 5], 1u << (j & 31));
+      if (f & HCX_FLAG_REAL_REF) ++c_rref;
+    }
     c_inf += (f & HCX_FLAG_INFINITY) ? 1 : 0;
     if (f & HCX_FLAG_REFINED) {
       ++c_ref;
       c_real += (f & HCX_FLAG_REAL) ? 1 : 0;
-      const double2* xj = roots + (base + j) * n;
       bool dup = false;
-      for (int i = 0; i < j && !dup; ++i) {
-        if (!(flags[base + i] & HCX_FLAG_REFINED)) continue;
-        const double2* xi = roots + (base + i) * n;
-        bool same = true;
-        for (int v = 0; v < n && same; ++v)
-          same = fabs(xi[v].x - xj[v].x) < tol && fabs(xi[v].y - xj[v].y) < tol;
-        dup = same;
-      }
+      for (int i = 0; i < j && !dup; ++i)
+        dup = (flags[base + i] & HCX_FLAG_REFINED) && same_root(roots + (base + i) * n, xj, n, tol);
       if (!dup) {
         f |= HCX_FLAG_UNIQUE;
         ++c_uni;
       }
     }
+    bool lat = false;
+    for (int i = j + 1; i < K && !lat; ++i) lat = same_root(roots + (base + i) * n, xj, n, tol);
+    if (lat) atomicOr(&later[j >> 5], 1u << (j & 31));
     flags[base + j] = f;
   }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long c_uref = 0;
+    int bstar = -1;
+    for (int bs = 0; bs < K; ++bs) {
+      if (!((tracked[bs >> 5] >> (bs & 31)) & 1u)) continue;
+      if (bstar >= 0 && same_root(roots + (base + bs) * n, roots + (base + bstar) * n, n, tol)) continue;
+      if ((later[bs >> 5] >> (bs & 31)) & 1u) {
+        bstar = bs;
+      } else {
+        uref[bs >> 5] |= 1u << (bs & 31);
+        ++c_uref;
+      }
+    }
+    atomicAdd(&counts[5], c_uref);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < K; j += blockDim.x)
+    if ((uref[j >> 5] >> (j & 31)) & 1u) flags[base + j] |= HCX_FLAG_UNIQUE_REF;
   atomicAdd(&counts[0], c_tr);
   atomicAdd(&counts[1], c_inf);
   atomicAdd(&counts[2], c_ref);
   atomicAdd(&counts[3], c_real);
   atomicAdd(&counts[4], c_uni);
+  atomicAdd(&counts[6], c_rref);
 }
 
 // Lists the paths of a mixed-precision launch that the FP64 pass re-tracks
@@ -281,9 +324,10 @@ namespace hcx {
 namespace {
 
 std::mutex g_mu;
-unsigned long long* g_counts = nullptr;   // device [5]
-unsigned long long* g_stamp = nullptr;    // device [1]
+unsigned long long* g_counts = nullptr;   // device [kNumCounts], hcx_collect's totals
 int g_cu = 0;
+constexpr int kNumCounts = 7;
+static_assert(kSegSlots == kMaxSeg, "workspace blocks hold one counter slot per segment");
 
 #define HIP_TRY(expr)                                                              \
   do {                                                                             \
@@ -294,17 +338,57 @@ int g_cu = 0;
     }                                                                              \
   } while (0)
 
-hcx_status ensure_device_globals() {
+// Compute-unit count of the device (sizes persistent grids); allocates nothing.
+hcx_status ensure_device_info() {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (g_counts) return HCX_OK;
+  if (g_cu > 0) return HCX_OK;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, dev));
   g_cu = prop.multiProcessorCount;
-  HIP_TRY(hipMalloc(&g_counts, 5 * sizeof(unsigned long long)));
-  HIP_TRY(hipMalloc(&g_stamp, sizeof(unsigned long long)));
   return HCX_OK;
+}
+
+// Device-memory policy of the workspace pool.
+struct HipDev {
+  void* alloc(size_t n) {
+    void* p = nullptr;
+    return hipMalloc(&p, n) == hipSuccess ? p : nullptr;
+  }
+  void release(void* p) { (void)hipFree(p); }
+  bool sync(const void* s) { return hipStreamSynchronize((hipStream_t)s) == hipSuccess; }
+  bool capturing(const void* s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing((hipStream_t)s, &st) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    return st == hipStreamCaptureStatusActive;
+  }
+};
+
+// Per-stream launch workspaces; lives until the process ends (blocks that
+// captured graphs refer to must outlive every graph).
+WorkspacePool<HipDev>& pool() {
+  static auto* p = new WorkspacePool<HipDev>(HipDev{}, sizeof(SegOut));
+  return *p;
+}
+
+hcx_status ws_error(WsStatus w, const char* who) {
+  switch (w) {
+    case WsStatus::kOk: return HCX_OK;
+    case WsStatus::kCaptureTooSmall:
+      set_error("%s: the stream's launch workspace is too small and cannot grow during graph capture; "
+                "run the same launch once on this stream before capturing it", who);
+      return HCX_ERR_INVALID_ARG;
+    case WsStatus::kOutOfMemory:
+      set_error("%s: out of device memory for the launch workspace", who);
+      return HCX_ERR_OUT_OF_MEMORY;
+    default:
+      set_error("%s: waiting for the stream before growing its workspace failed", who);
+      return HCX_ERR_HIP;
+  }
 }
 
 int round_up_idx(const int* list, int cnt, int v) {
@@ -348,6 +432,7 @@ Opts to_opts(const hcx_track_opts* o) {
   r.inf_tol = o->inf_tol;
   r.refine_tol = o->refine_tol;
   r.real_tol = o->real_tol;
+  r.real_ref_tol = o->real_ref_tol;
   r.newton32 = o->fp32_corrector ? 1 : 0;
   return r;
 }
@@ -428,7 +513,6 @@ hcx_status choose_kernel(int nseg, const hcx_problem* const* probs, const int* G
 void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, int nmax) {
   std::memset(&S, 0, sizeof(S));
   std::memset(&O, 0, sizeof(O));
-  S.out = p->d_out;
   O.roots_stride = p->n;
   S.n = p->n;
   S.C = p->C;
@@ -446,7 +530,6 @@ void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, 
   S.ctab = p->d_ctab;
   S.start = p->d_start;
   S.start_stride = p->n;
-  S.counter = p->d_counter;
 }
 
 hcx_status finish_plan(Launch& LN, Plan& pl) {
@@ -480,28 +563,67 @@ int default_lanes(int n, int C, int M) {
   return G;
 }
 
-hcx_status ensure_plist_cap(hcx_problem* p, long long P) {
-  if (!p->d_plist_count) HIP_TRY(hipMalloc(&p->d_plist_count, sizeof(unsigned int)));
-  if (p->plist_cap >= P) return HCX_OK;
-  if (p->d_plist) (void)hipFree(p->d_plist);
-  p->d_plist = nullptr;
-  p->plist_cap = 0;
-  HIP_TRY(hipMalloc(&p->d_plist, sizeof(int) * (size_t)std::max<long long>(P, 1)));
-  p->plist_cap = std::max<long long>(P, 1);
-  return HCX_OK;
+// Workspace entries a launch needs: instance counters when instance
+// completion times are requested, the re-track list in mixed precision.
+void ws_need(int nseg, hcx_problem* const* probs, const hcx_batch_desc* batches, const hcx_result_desc* results,
+             const hcx_track_opts* o, long long& inst, long long& paths) {
+  inst = paths = 0;
+  for (int s = 0; s < nseg; ++s) {
+    if (!probs[s] || batches[s].num_instances < 0) continue;
+    const long long I = batches[s].num_instances;
+    if (results[s].d_inst_done) inst += I;
+    if (o && o->precision == HCX_PREC_MIXED) paths += I * probs[s]->K;
+  }
 }
 
-hcx_status ensure_inst_cap(hcx_problem* p, int I) {
-  if (p->inst_cap >= I) return HCX_OK;
-  if (p->d_inst_left) (void)hipFree(p->d_inst_left);
-  p->d_inst_left = nullptr;
-  p->inst_cap = 0;
-  HIP_TRY(hipMalloc(&p->d_inst_left, sizeof(int) * (size_t)std::max(I, 1)));
-  p->inst_cap = std::max(I, 1);
-  return HCX_OK;
+// Carves the per-segment views of a block for the given launch.
+void ws_carve(const WsBlock& b, int nseg, const hcx_batch_desc* batches, const hcx_result_desc* results,
+              hcx_problem* const* probs, const hcx_track_opts* o, LaunchWork* work) {
+  long long io = 0, po = 0;
+  for (int s = 0; s < nseg; ++s) {
+    work[s] = seg_work(b, s, io, po);
+    const long long I = std::max(batches[s].num_instances, 0);
+    if (results[s].d_inst_done) io += I;
+    if (o->precision == HCX_PREC_MIXED && probs[s]) po += I * probs[s]->K;
+  }
 }
 
 }  // namespace
+
+LaunchWork seg_work(const WsBlock& b, int s, long long inst_off, long long path_off) {
+  const WsLayout L = ws_layout(sizeof(SegOut), b.inst_cap, b.paths_cap);
+  char* base = static_cast<char*>(b.base);
+  LaunchWork w;
+  w.counter = reinterpret_cast<unsigned int*>(base) + 64 * s;
+  w.plist_count = w.counter + 32;
+  w.out = reinterpret_cast<SegOut*>(base + L.out_off) + s;
+  w.inst_left = reinterpret_cast<int*>(base + L.inst_off) + inst_off;
+  w.plist = reinterpret_cast<int*>(base + L.plist_off) + path_off;
+  w.inst_cap = b.inst_cap - inst_off;
+  w.plist_cap = b.paths_cap - path_off;
+  return w;
+}
+
+// A launch on `stream` with that stream's pooled workspace.
+hcx_status track_pooled(int nseg, hcx_problem* const* probs, const hcx_batch_desc* batches,
+                        const hcx_result_desc* results, const hcx_track_opts* o, hipStream_t stream, int persistent,
+                        unsigned long long* batch_start, const char* who) {
+  if (nseg < 1 || nseg > kMaxSeg || !probs || !batches || !results || !o) {
+    set_error("%s: need 1..%d segments with problems, batches, results and options", who, kMaxSeg);
+    return HCX_ERR_INVALID_ARG;
+  }
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  long long inst = 0, paths = 0;
+  ws_need(nseg, probs, batches, results, o, inst, paths);
+  WorkspacePool<HipDev>::Lease lease;
+  hcx_status st = ws_error(pool().acquire(dev, stream, inst, paths, lease), who);
+  if (st != HCX_OK) return st;
+  LaunchWork work[kMaxSeg];
+  ws_carve(lease.block(), nseg, batches, results, probs, o, work);
+  return track_impl(nseg, probs, batches, results, o, stream, persistent, batch_start, work);
+}
+
 }  // namespace hcx
 
 using namespace hcx;
@@ -550,7 +672,7 @@ hcx_status hcx_problem_create(const hcx_problem_desc* d, hcx_problem** out) {
     set_error("hcx_problem_create: no HIP device available");
     return HCX_ERR_NO_DEVICE;
   }
-  hcx_status st = ensure_device_globals();
+  hcx_status st = ensure_device_info();
   if (st != HCX_OK) return st;
 
   const int hxv = d->hx_parts - 2, htv = d->ht_parts - 2;
@@ -686,8 +808,8 @@ hcx_status hcx_problem_create(const hcx_problem_desc* d, hcx_problem** out) {
       return fail(e, "copy ctab");
   }
   if ((e = hipMalloc(&p->d_start, (size_t)p->K * n * sizeof(double2))) != hipSuccess) return fail(e, "hipMalloc start");
-  if ((e = hipMalloc(&p->d_counter, 64 * sizeof(unsigned int))) != hipSuccess) return fail(e, "hipMalloc counter");
-  if ((e = hipMalloc(&p->d_out, sizeof(SegOut))) != hipSuccess) return fail(e, "hipMalloc out");
+  if ((e = hipMalloc(&p->launcher_ws.base, ws_layout(sizeof(SegOut), 0, 0).bytes)) != hipSuccess)
+    return fail(e, "hipMalloc launcher workspace");
   if ((e = hipMemcpy(p->d_hx, hx.data(), hx.size() * sizeof(dev::Term), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(e, "copy hx");
   if ((e = hipMemcpy(p->d_ht, ht.data(), ht.size() * sizeof(dev::Term), hipMemcpyHostToDevice)) != hipSuccess)
@@ -776,11 +898,7 @@ hcx_status hcx_problem_destroy(hcx_problem* p) {
   if (p->d_p0) (void)hipFree(p->d_p0);
   if (p->d_cmap_scalar) (void)hipFree(p->d_cmap_scalar);
   if (p->d_cmap_params) (void)hipFree(p->d_cmap_params);
-  if (p->d_counter) (void)hipFree(p->d_counter);
-  if (p->d_out) (void)hipFree(p->d_out);
-  if (p->d_inst_left) (void)hipFree(p->d_inst_left);
-  if (p->d_plist) (void)hipFree(p->d_plist);
-  if (p->d_plist_count) (void)hipFree(p->d_plist_count);
+  if (p->launcher_ws.base) (void)hipFree(p->launcher_ws.base);
   delete p;
   return HCX_OK;
 }
@@ -801,23 +919,34 @@ hcx_status hcx_sync(void* stream) {
   return HCX_OK;
 }
 
+hcx_status hcx_stream_release(void* stream) {
+  clear_error();
+  if (hcx_device_count() < 1) {
+    set_error("hcx_stream_release: no HIP device available");
+    return HCX_ERR_NO_DEVICE;
+  }
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  return ws_error(pool().release_stream(dev, stream), "hcx_stream_release");
+}
+
 // Shared implementation of hcx_track / hcx_track_multi.
 }  // extern "C"
 
 namespace hcx {
 // Shared implementation of hcx_track / hcx_track_multi / graphs / the server.
-// `work` (optional, one per segment) replaces the problem-owned launch
-// workspace so that several launches of one problem can be in flight.
+// `work` (one per segment) is the launch workspace: the stream's pooled
+// block, a graph's own block, a server slot's block or the static scratch.
 hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc* batches,
                       const hcx_result_desc* results, const hcx_track_opts* o, hipStream_t stream, int persistent,
                       unsigned long long* batch_start, const LaunchWork* work) {
   hcx_status st = validate_opts(o);
   if (st != HCX_OK) return st;
-  if (nseg < 1 || nseg > kMaxSeg || !probs || !batches || !results) {
-    set_error("track: need 1..%d segments with problems, batches and results", kMaxSeg);
+  if (nseg < 1 || nseg > kMaxSeg || !probs || !batches || !results || !work) {
+    set_error("track: need 1..%d segments with problems, batches, results and a workspace", kMaxSeg);
     return HCX_ERR_INVALID_ARG;
   }
-  st = ensure_device_globals();
+  st = ensure_device_info();
   if (st != HCX_OK) return st;
   Launch LN;
   std::memset(&LN, 0, sizeof(LN));
@@ -920,14 +1049,16 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
     O.dtfin = r.d_dt_final;
     O.resid = r.d_residual;
     O.inst_done = reinterpret_cast<unsigned long long*>(r.d_inst_done);
-    if (work) {
-      S.counter = work[s].counter;
-      S.out = work[s].out;
-      O.inst_left = r.d_inst_done ? work[s].inst_left : nullptr;
-    } else if (r.d_inst_done) {
-      st = ensure_inst_cap(p, b.num_instances);
-      if (st != HCX_OK) return st;
-      O.inst_left = p->d_inst_left;
+    S.counter = work[s].counter;
+    S.out = work[s].out;
+    O.inst_left = nullptr;
+    if (r.d_inst_done && b.num_instances > 0) {
+      if (!work[s].inst_left || work[s].inst_cap < b.num_instances) {
+        set_error("track: launch workspace instance counters too small (%lld < %d)", work[s].inst_cap,
+                  b.num_instances);
+        return HCX_ERR_INVALID_ARG;
+      }
+      O.inst_left = work[s].inst_left;
     }
     const int gpw = 64 / G;
     seg_groups[s] = (S.P + gpw - 1) / gpw;
@@ -992,21 +1123,13 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
     for (int s = 0; s < nseg; ++s) {
       hcx_problem* p = probs[s];
       Seg& S = L2.seg[s];
-      int* plist;
-      unsigned int* pcount;
-      if (work) {
-        if (work[s].plist_cap < S.P || !work[s].plist || !work[s].plist_count) {
-          set_error("track: launch workspace path list too small (%lld < %lld)", work[s].plist_cap, S.P);
-          return HCX_ERR_INVALID_ARG;
-        }
-        plist = work[s].plist;
-        pcount = work[s].plist_count;
-      } else {
-        st = ensure_plist_cap(p, S.P);
-        if (st != HCX_OK) return st;
-        plist = p->d_plist;
-        pcount = p->d_plist_count;
+      (void)p;
+      if (work[s].plist_cap < S.P || !work[s].plist || !work[s].plist_count) {
+        set_error("track: launch workspace path list too small (%lld < %lld)", work[s].plist_cap, S.P);
+        return HCX_ERR_INVALID_ARG;
       }
+      int* plist = work[s].plist;
+      unsigned int* pcount = work[s].plist_count;
       hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, pcount, 1);
       if (S.I > 0)
         hipLaunchKernelGGL(dup_mark_kernel, dim3(S.I), dim3(256), 0, stream, S.n, S.K,
@@ -1057,7 +1180,8 @@ hcx_status hcx_track(hcx_problem* p, const hcx_batch_desc* b, const hcx_track_op
     set_error("hcx_track: null argument");
     return HCX_ERR_INVALID_ARG;
   }
-  return track_impl(1, &p, b, r, o, (hipStream_t)stream, 1, reinterpret_cast<unsigned long long*>(r->d_batch_start), nullptr);
+  return track_pooled(1, &p, b, r, o, (hipStream_t)stream, 1, reinterpret_cast<unsigned long long*>(r->d_batch_start),
+                      "hcx_track");
 }
 
 hcx_status hcx_track_multi(int num_segments, hcx_problem* const* problems, const hcx_batch_desc* batches,
@@ -1067,8 +1191,8 @@ hcx_status hcx_track_multi(int num_segments, hcx_problem* const* problems, const
     set_error("hcx_track_multi: no segments");
     return HCX_ERR_INVALID_ARG;
   }
-  return track_impl(num_segments, problems, batches, results, o, (hipStream_t)stream, 1,
-                    reinterpret_cast<unsigned long long*>(results[0].d_batch_start), nullptr);
+  return track_pooled(num_segments, problems, batches, results, o, (hipStream_t)stream, 1,
+                      reinterpret_cast<unsigned long long*>(results[0].d_batch_start), "hcx_track_multi");
 }
 
 double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, int max_corr, int incr_steps,
@@ -1089,8 +1213,12 @@ double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, 
   o.dt_min = 0.0;
   o.refine_iters = 0;
   if (validate_opts(&o) != HCX_OK) return -1.0;
-  if (ensure_device_globals() != HCX_OK) return -1.0;
+  if (ensure_device_info() != HCX_OK) return -1.0;
   hipStream_t s = (hipStream_t)stream;
+  // the problem's launcher block (counters, output descriptor) serves one
+  // synchronous call at a time
+  std::lock_guard<std::mutex> lk(p->launcher_mu);
+  const LaunchWork w = seg_work(p->launcher_ws, 0, 0, 0);
   Launch LN;
   std::memset(&LN, 0, sizeof(LN));
   OutPack OP;
@@ -1112,20 +1240,10 @@ double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, 
   S.P = p->K;
   S.chx = reinterpret_cast<const double2*>(d_coeffs_hx);
   S.cht = reinterpret_cast<const double2*>(d_coeffs_ht);
-  // flags are needed by the kernel; reuse the problem's scratch via a small buffer
-  static thread_local uint32_t* flags_buf = nullptr;
-  static thread_local int flags_cap = 0;
-  if (flags_cap < p->K) {
-    if (flags_buf) (void)hipFree(flags_buf);
-    if (hipMalloc(&flags_buf, sizeof(uint32_t) * p->K) != hipSuccess) {
-      set_error("hcx_kernel_HC_Solver: hipMalloc flags failed");
-      flags_cap = 0;
-      flags_buf = nullptr;
-      return -1.0;
-    }
-    flags_cap = p->K;
-  }
-  O.flags = flags_buf;
+  S.counter = w.counter;
+  S.out = w.out;
+  // the reference's outputs only: converged / infinity bytes and (t, dt)
+  O.flags = nullptr;
   O.conv8 = d_converge;
   O.inf8 = d_infinity;
   O.debug = reinterpret_cast<double2*>(d_debug);
@@ -1164,7 +1282,8 @@ hcx_status hcx_track_static(hcx_problem* p, const hcx_batch_desc* b, const hcx_t
     set_error("hcx_track_static: null argument");
     return HCX_ERR_INVALID_ARG;
   }
-  return track_impl(1, &p, b, r, o, (hipStream_t)stream, 0, reinterpret_cast<unsigned long long*>(r->d_batch_start), nullptr);
+  return track_pooled(1, &p, b, r, o, (hipStream_t)stream, 0, reinterpret_cast<unsigned long long*>(r->d_batch_start),
+                      "hcx_track_static");
 }
 
 hcx_status hcx_collect(int n, int num_instances, int num_tracks, const hcx_complex* d_roots, uint32_t* d_flags,
@@ -1175,16 +1294,22 @@ hcx_status hcx_collect(int n, int num_instances, int num_tracks, const hcx_compl
     set_error("hcx_collect: invalid arguments");
     return HCX_ERR_INVALID_ARG;
   }
-  hcx_status st = ensure_device_globals();
+  const size_t bit_lds = 3 * sizeof(unsigned int) * (size_t)((num_tracks + 31) / 32);
+  if (bit_lds > 65536) {
+    set_error("hcx_collect: %d paths per instance exceed the collection kernel's bit sets", num_tracks);
+    return HCX_ERR_UNSUPPORTED;
+  }
+  hcx_status st = ensure_device_info();
   if (st != HCX_OK) return st;
   std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_counts) HIP_TRY(hipMalloc(&g_counts, kNumCounts * sizeof(unsigned long long)));
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, g_counts, 5);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, g_counts, kNumCounts);
   if (!empty)
-    hipLaunchKernelGGL(collect_kernel, dim3(num_instances), dim3(256), 0, s, n, num_tracks,
+    hipLaunchKernelGGL(collect_kernel, dim3(num_instances), dim3(256), bit_lds, s, n, num_tracks,
                        reinterpret_cast<const double2*>(d_roots), d_flags, unique_tol, g_counts);
   HIP_TRY(hipGetLastError());
-  unsigned long long h[5];
+  unsigned long long h[kNumCounts];
   HIP_TRY(hipMemcpyAsync(h, g_counts, sizeof(h), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   out->tracked = (int64_t)h[0];
@@ -1192,6 +1317,8 @@ hcx_status hcx_collect(int n, int num_instances, int num_tracks, const hcx_compl
   out->refined = (int64_t)h[2];
   out->real = (int64_t)h[3];
   out->unique = (int64_t)h[4];
+  out->unique_ref = (int64_t)h[5];
+  out->real_ref = (int64_t)h[6];
   return HCX_OK;
 }
 
@@ -1266,11 +1393,12 @@ hcx_status hcx_autotune(hcx_problem* p, const hcx_batch_desc* sample, const hcx_
     oc.lanes_per_path = c.G;
     oc.specialize = c.spec;
     oc.waves_per_cu = c.wpc;
-    st = track_impl(1, &p, sample, &r, &oc, s, 1, nullptr, nullptr);  // warm-up
+    st = track_pooled(1, &p, sample, &r, &oc, s, 1, nullptr, "hcx_autotune");  // warm-up
     if (st != HCX_OK) continue;
     HIP_TRY(hipEventRecord(e0, s));
     const int reps = 2;
-    for (int q = 0; q < reps && st == HCX_OK; ++q) st = track_impl(1, &p, sample, &r, &oc, s, 1, nullptr, nullptr);
+    for (int q = 0; q < reps && st == HCX_OK; ++q)
+      st = track_pooled(1, &p, sample, &r, &oc, s, 1, nullptr, "hcx_autotune");
     HIP_TRY(hipEventRecord(e1, s));
     HIP_TRY(hipEventSynchronize(e1));
     if (st != HCX_OK) continue;
@@ -1326,45 +1454,77 @@ hcx_status hcx_problem_set_launch_shape(hcx_problem* p, int lanes_per_path, int 
 struct hcx_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // the graph's own launch workspace (path queues, output descriptors,
+  // instance counters, re-track list): baked into the graph, freed with it
+  WsBlock ws;
+  hipEvent_t last = nullptr;  // completion of the latest replay
 };
 
 extern "C" {
 
 // Captures one multi-segment persistent launch (counter set-up, device P2C,
-// tracking) into a HIP graph.  Counters are cleared by a kernel, not a memset
-// node.  The arguments are baked into the graph: the same device buffers are
-// reused on every hcx_graph_launch.
+// tracking, and in mixed precision the FP64 re-track pass) into a HIP graph.
+// Counters are cleared by kernels, not memset nodes.  The arguments are baked
+// into the graph: the same device buffers are reused on every
+// hcx_graph_launch.  The launch workspace belongs to the graph, so eager
+// launches of the same problems on any stream, of any size, never touch the
+// memory a replay uses.
 hcx_status hcx_graph_capture_multi(int num_segments, hcx_problem* const* problems, const hcx_batch_desc* batches,
                                    const hcx_result_desc* results, const hcx_track_opts* o, hcx_graph** out) {
   clear_error();
-  if (!out || num_segments < 1 || !results) {
+  if (!out || num_segments < 1 || num_segments > kMaxSeg || !problems || !batches || !results || !o) {
     set_error("hcx_graph_capture_multi: invalid arguments");
     return HCX_ERR_INVALID_ARG;
   }
-  // an uncaptured run first: allocates per-problem workspace and sets kernel
-  // attributes outside the capture
-  hipStream_t cs = nullptr;
-  HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-  hcx_status st = track_impl(num_segments, problems, batches, results, o, cs, 1,
-                             reinterpret_cast<unsigned long long*>(results[0].d_batch_start), nullptr);
-  if (st != HCX_OK) {
-    (void)hipStreamDestroy(cs);
-    return st;
-  }
-  HIP_TRY(hipStreamSynchronize(cs));
+  for (int s = 0; s < num_segments; ++s)
+    if (!problems[s]) {
+      set_error("hcx_graph_capture_multi: segment %d has no problem", s);
+      return HCX_ERR_INVALID_ARG;
+    }
   auto* g = new hcx_graph();
-  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-  if (e == hipSuccess) {
+  long long inst = 0, paths = 0;
+  ws_need(num_segments, problems, batches, results, o, inst, paths);
+  const WsLayout L = ws_layout(sizeof(SegOut), inst, paths);
+  hipError_t e = hipMalloc(&g->ws.base, L.bytes);
+  if (e != hipSuccess) {
+    delete g;
+    set_error("hcx_graph_capture_multi: workspace allocation: %s", hipGetErrorString(e));
+    return HCX_ERR_OUT_OF_MEMORY;
+  }
+  g->ws.inst_cap = inst;
+  g->ws.paths_cap = paths;
+  g->ws.pinned = true;
+  LaunchWork work[kMaxSeg];
+  ws_carve(g->ws, num_segments, batches, results, problems, o, work);
+  // an uncaptured run first: sets kernel attributes outside the capture
+  hipStream_t cs = nullptr;
+  hcx_status st = HCX_OK;
+  if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&g->last, hipEventDisableTiming)) != hipSuccess) {
+    set_error("hcx_graph_capture_multi: %s", hipGetErrorString(e));
+    st = HCX_ERR_HIP;
+  }
+  if (st == HCX_OK)
     st = track_impl(num_segments, problems, batches, results, o, cs, 1,
-                    reinterpret_cast<unsigned long long*>(results[0].d_batch_start), nullptr);
-    hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
-    if (st == HCX_OK && e2 != hipSuccess) {
-      set_error("hipStreamEndCapture: %s", hipGetErrorString(e2));
+                    reinterpret_cast<unsigned long long*>(results[0].d_batch_start), work);
+  if (st == HCX_OK && (e = hipStreamSynchronize(cs)) != hipSuccess) {
+    set_error("hcx_graph_capture_multi: %s", hipGetErrorString(e));
+    st = HCX_ERR_HIP;
+  }
+  if (st == HCX_OK) {
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      st = track_impl(num_segments, problems, batches, results, o, cs, 1,
+                      reinterpret_cast<unsigned long long*>(results[0].d_batch_start), work);
+      hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
+      if (st == HCX_OK && e2 != hipSuccess) {
+        set_error("hipStreamEndCapture: %s", hipGetErrorString(e2));
+        st = HCX_ERR_HIP;
+      }
+    } else {
+      set_error("hipStreamBeginCapture: %s", hipGetErrorString(e));
       st = HCX_ERR_HIP;
     }
-  } else {
-    set_error("hipStreamBeginCapture: %s", hipGetErrorString(e));
-    st = HCX_ERR_HIP;
   }
   if (st == HCX_OK) {
     e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
@@ -1373,7 +1533,7 @@ hcx_status hcx_graph_capture_multi(int num_segments, hcx_problem* const* problem
       st = HCX_ERR_HIP;
     }
   }
-  (void)hipStreamDestroy(cs);
+  if (cs) (void)hipStreamDestroy(cs);
   if (st != HCX_OK) {
     hcx_graph_destroy(g);
     return st;
@@ -1389,13 +1549,20 @@ hcx_status hcx_graph_launch(hcx_graph* g, void* stream) {
     return HCX_ERR_INVALID_ARG;
   }
   HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  HIP_TRY(hipEventRecord(g->last, (hipStream_t)stream));
   return HCX_OK;
 }
 
+// Waits for the latest replay, then frees the graph and its workspace.
 hcx_status hcx_graph_destroy(hcx_graph* g) {
   if (!g) return HCX_OK;
+  if (g->last) {
+    (void)hipEventSynchronize(g->last);
+    (void)hipEventDestroy(g->last);
+  }
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->ws.base) (void)hipFree(g->ws.base);
   delete g;
   return HCX_OK;
 }

@@ -22,6 +22,8 @@ FLAG_BUDGET = 0x20
 FLAG_MINSTEP = 0x40
 FLAG_SINGULAR = 0x80
 FLAG_ESCALATED = 0x100
+FLAG_REAL_REF = 0x200
+FLAG_UNIQUE_REF = 0x400
 
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -247,7 +249,31 @@ class Problem:
         self.close()
 
 
-class Result:
+class _DeviceBuffers:
+    """Device tensors allocated (and zero-filled or uploaded) on the stream
+    that was current at construction.  A launch on another stream first
+    waits for that initialisation (an event recorded at construction, so the
+    wait does not order the launch behind later, unrelated work), and the
+    tensors are recorded as used by the launch stream, so the caching
+    allocator does not hand them out while the launch still runs."""
+
+    def _mark_ready(self):
+        import torch
+        self._ready = torch.cuda.Event()
+        self._ready.record(torch.cuda.current_stream())
+        self._streams = set()
+
+    def _use_on(self, s):
+        key = s.cuda_stream
+        if key in self._streams:
+            return
+        s.wait_event(self._ready)
+        for t in self.tensors():
+            t.record_stream(s)
+        self._streams.add(key)
+
+
+class Result(_DeviceBuffers):
     """Per-path device outputs of one segment (torch tensors on cuda)."""
 
     def __init__(self, P, n, I, device="cuda"):
@@ -263,6 +289,11 @@ class Result:
         self.residual = torch.zeros(max(P, 1), dtype=torch.float64, device=device)
         self.inst_done = torch.zeros(max(I, 1), dtype=torch.int64, device=device)
         self.batch_start = torch.zeros(1, dtype=torch.int64, device=device)
+        self._mark_ready()
+
+    def tensors(self):
+        return (self.roots, self.flags, self.steps, self.newton, self.refine, self.t_final, self.dt_final,
+                self.residual, self.inst_done, self.batch_start)
 
     def desc(self):
         r = ResultDesc()
@@ -290,7 +321,7 @@ class Result:
         return out
 
 
-class Batch:
+class Batch(_DeviceBuffers):
     """Instances of one problem: coefficient tables or target parameters on device."""
 
     def __init__(self, problem, num_instances, coeffs_hx=None, coeffs_ht=None, target_params=None,
@@ -316,6 +347,10 @@ class Batch:
         if self.coeffs_hx is None:
             self.work = torch.zeros(max(num_instances, 1) * (problem.num_coeffs + 1) * (2 * problem.max_order_t + 1),
                                     dtype=torch.complex128, device=device)
+        self._mark_ready()
+
+    def tensors(self):
+        return tuple(self._t) + ((self.work,) if self.work is not None else ())
 
     def desc(self):
         b = BatchDesc()
@@ -328,9 +363,15 @@ class Batch:
         return b
 
 
-def _stream_handle(stream):
+def _stream_handle(stream, uses=()):
+    """HIP stream of a launch (torch's current stream when None); the
+    buffers in `uses` are made safe for it (see _DeviceBuffers)."""
     import torch
-    s = stream if stream is not None else torch.cuda.current_stream()
+    cur = torch.cuda.current_stream()
+    s = stream if stream is not None else cur
+    if s != cur:
+        for obj in uses:
+            obj._use_on(s)
     return C.c_void_p(s.cuda_stream)
 
 
@@ -340,7 +381,8 @@ def track(problem, batch, opts, result=None, static=False, stream=None):
         result = Result(batch.I * problem.num_tracks, problem.n, batch.I)
     b, r = batch.desc(), result.desc()
     fn = lib().hcx_track_static if static else lib().hcx_track
-    _check(fn(problem.handle, C.byref(b), C.byref(opts), C.byref(r), _stream_handle(stream)), "hcx_track")
+    _check(fn(problem.handle, C.byref(b), C.byref(opts), C.byref(r), _stream_handle(stream, (batch, result))),
+           "hcx_track")
     return result
 
 
@@ -351,7 +393,8 @@ def track_multi(problems, batches, opts, results=None, stream=None):
     ph = (C.c_void_p * ns)(*[p.handle for p in problems])
     bd = (BatchDesc * ns)(*[b.desc() for b in batches])
     rd = (ResultDesc * ns)(*[r.desc() for r in results])
-    _check(lib().hcx_track_multi(ns, ph, bd, rd, C.byref(opts), _stream_handle(stream)), "hcx_track_multi")
+    _check(lib().hcx_track_multi(ns, ph, bd, rd, C.byref(opts), _stream_handle(stream, list(batches) + list(results))),
+           "hcx_track_multi")
     return results
 
 
@@ -382,7 +425,8 @@ class Graph:
         self._h = h
 
     def launch(self, stream=None):
-        _check(lib().hcx_graph_launch(self._h, _stream_handle(stream)), "hcx_graph_launch")
+        _check(lib().hcx_graph_launch(self._h, _stream_handle(stream, list(self.batches) + list(self.results))),
+               "hcx_graph_launch")
         return self.results
 
     def close(self):
@@ -466,8 +510,13 @@ def collect(result, opts, stream=None):
     c = Counts()
     _check(lib().hcx_collect(result.n, result.I, result.P // max(result.I, 1), C.c_void_p(result.roots.data_ptr()),
                              C.c_void_p(result.flags.data_ptr()), opts.unique_tol, C.byref(c),
-                             _stream_handle(stream)), "hcx_collect")
+                             _stream_handle(stream, (result,))), "hcx_collect")
     return {f: getattr(c, f) for f, _ in Counts._fields_}
+
+
+def stream_release(stream):
+    """Frees the launch workspace libhcx keeps for `stream` (after waiting for it)."""
+    _check(lib().hcx_stream_release(C.c_void_p(stream.cuda_stream)), "hcx_stream_release")
 
 
 def collect_host(n, num_instances, num_tracks, roots, flags, unique_tol=1e-4):

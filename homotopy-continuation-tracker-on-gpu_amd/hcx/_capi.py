@@ -18,6 +18,7 @@ class TrackOpts(C.Structure):
         ("refine_iters", C.c_int), ("refine_tol", C.c_double), ("real_tol", C.c_double),
         ("unique_tol", C.c_double), ("precision", C.c_int), ("lanes_per_path", C.c_int),
         ("waves_per_cu", C.c_int), ("specialize", C.c_int), ("retrack_duplicates", C.c_int), ("fp32_corrector", C.c_int),
+        ("real_ref_tol", C.c_double),
     ]
 
 
@@ -61,7 +62,7 @@ class ResultDesc(C.Structure):
 
 class Counts(C.Structure):
     _fields_ = [("tracked", C.c_int64), ("infinity", C.c_int64), ("refined", C.c_int64), ("real", C.c_int64),
-                ("unique", C.c_int64)]
+                ("unique", C.c_int64), ("unique_ref", C.c_int64), ("real_ref", C.c_int64)]
 
 
 class SynthSpec(C.Structure):
@@ -125,6 +126,7 @@ SIGNATURES = {
                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p]),
     "hcx_sync": (C.c_int, [C.c_void_p]),
+    "hcx_stream_release": (C.c_int, [C.c_void_p]),
     "hcx_autotune": (C.c_int, [C.c_void_p, C.POINTER(BatchDesc), C.POINTER(TrackOpts), C.POINTER(TuneResult),
                                C.c_void_p]),
     "hcx_problem_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),

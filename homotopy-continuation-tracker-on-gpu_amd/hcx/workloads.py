@@ -10,7 +10,8 @@ homotopy maps start roots 1-to-1 onto target roots.
 
 Start roots of a parameter homotopy come from a seed instance p0 solved by
 total-degree homotopy on a seeded subset of the prod d_i start roots; the first
-K refined, distinct roots (in subset order) are kept.
+K refined, distinct roots (in subset order) are kept; further rounds (new
+subset, new gamma) run until K distinct roots are found.
 """
 import numpy as np
 
@@ -68,10 +69,12 @@ def seed_start_roots(system, p0, K, seed, opts=None, margin=1.3, max_rounds=4):
     if opts is None:
         opts = default_opts(precision=PREC_FP64, max_steps=400)
     coeffs = system.eval_coeffs(p0)
-    gamma = random_gamma(_seed(seed, 11))
     count = K
     found = np.zeros((0, system.info["num_vars"]), dtype=np.complex128)
     for rnd in range(max_rounds):
+        # a new gamma each round: when K is the whole Bezout number a path
+        # that fails for one gamma is tracked again along another path
+        gamma = random_gamma(_seed(seed, 11) if rnd == 0 else _seed(seed, 1000 + rnd))
         want = int(np.ceil(count * margin)) + 16
         idx = system.subset(want, _seed(seed, 13 + rnd))
         idx = idx[idx >= 0]

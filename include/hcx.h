@@ -82,6 +82,8 @@ typedef struct {
                              FP64 tracker's (1, default) */
   int fp32_corrector;     /* mixed precision: 1 = Newton corrector in FP32 before the end zone (FP64 in it);
                              0 (default) = FP64 corrector at every t, so step acceptance is decided in FP64 */
+  double real_ref_tol;    /* HCX_FLAG_REAL_REF: the reference's real test on the tracked root,
+                             ZERO_IMAG_PART_TOL_FOR_SP = 1e-4 (definitions.hpp:25, Evaluations.cpp:96-104) */
 } hcx_track_opts;
 
 void hcx_track_opts_default(hcx_track_opts* o);
@@ -183,9 +185,22 @@ typedef struct {
 } hcx_result_desc;
 
 /* Tracks every (instance, start root) path of a batch on `stream`.
- * Asynchronous; workspace is owned by the problem object. */
+ * Asynchronous.  The launch workspace (path-queue counters, output
+ * descriptors, per-instance counters, the FP64 re-track list) belongs to the
+ * stream: launches on one stream share it in stream order, launches on
+ * different streams never share it, so one problem may be tracked on several
+ * streams at once.  A stream's workspace grows, after waiting for the
+ * stream, when a launch needs more room; during graph capture it cannot grow
+ * (HCX_ERR_INVALID_ARG: run the same launch once uncaptured first), and a
+ * workspace used under capture is never freed or resized afterwards. */
 hcx_status hcx_track(hcx_problem* p, const hcx_batch_desc* b, const hcx_track_opts* o,
                      const hcx_result_desc* r, void* stream);
+
+/* Frees the launch workspace of `stream` after waiting for the stream (call
+ * before destroying a stream that tracked; a workspace that a captured graph
+ * refers to stays alive).  No counterpart in the reference, whose launcher
+ * uses no workspace (kernel_HC_Solver_generalized_3views_4pts.cu:299-300). */
+hcx_status hcx_stream_release(void* stream);
 
 /* Same work with the reference's launch shape: each lane group tracks exactly
  * one path and exits (grid = paths / groups per wave, no refill, no
@@ -206,7 +221,9 @@ hcx_status hcx_track_multi(int num_segments, hcx_problem* const* problems,
  * in kernel_HC_Solver_<p> (magmaHC-kernels.hpp:23-364).  d_start_sols and
  * d_track are [K][n+1] (last entry 1, Data_Reader.cpp:110-113), d_converge /
  * d_infinity are bool [K], d_debug is (t, dt) per path.  Returns the elapsed
- * seconds of the launch like the reference (<0 on error). */
+ * seconds of the launch like the reference (<0 on error).  Allocates nothing:
+ * its counters live with the problem (hcx_problem_create), and calls on one
+ * problem are serialised. */
 double hcx_kernel_HC_Solver(void* stream, const hcx_problem* p, int max_steps, int max_corr,
                             int incr_steps, const hcx_complex* d_start_sols, hcx_complex* d_track,
                             const hcx_complex* d_coeffs_hx, const hcx_complex* d_coeffs_ht,
@@ -297,14 +314,23 @@ hcx_status hcx_server_destroy(hcx_server* s);
 
 /* ------------------------------------------------------------------------ */
 /* Solution collection (GPU-HC/magmaHC/Evaluations.cpp:66-176).  Sets        */
-/* HCX_FLAG_UNIQUE per instance on device and returns host counts.           */
+/* HCX_FLAG_UNIQUE and HCX_FLAG_UNIQUE_REF per instance on device and        */
+/* returns host counts.                                                      */
 /* ------------------------------------------------------------------------ */
 typedef struct {
   int64_t tracked;    /* Num_Of_Coverged_Sols (reference semantics: t reached 1) */
   int64_t infinity;   /* Num_Of_Inf_Sols */
   int64_t refined;    /* root-level converged: tracked and FP64-refined */
-  int64_t real;       /* refined and real */
-  int64_t unique;     /* refined, distinct within their instance */
+  int64_t real;       /* refined and real (|Im x_i| <= real_tol on the refined root) */
+  int64_t unique;     /* refined, first occurrence of its root within the instance */
+  /* The reference's own integer outputs, per instance: */
+  int64_t unique_ref; /* Num_Of_Unique_Sols by Find_Unique_Sols (Evaluations.cpp:127-176): a tracked
+                         path counts when no later path of its instance (any outcome) has the same
+                         root (|dRe|, |dIm| < unique_tol in every coordinate), skipping paths that
+                         duplicate the last path found to have a later duplicate (the Skip set is
+                         replaced, not accumulated); a duplicate pair therefore adds 0.  Computed on
+                         the output roots: the tracked roots when refine_iters = 0. */
+  int64_t real_ref;   /* Num_Of_Real_Sols (Evaluations.cpp:88-110): paths with HCX_FLAG_REAL_REF */
 } hcx_counts;
 
 hcx_status hcx_collect(int n, int num_instances, int num_tracks, const hcx_complex* d_roots,

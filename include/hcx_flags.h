@@ -12,5 +12,9 @@
 #define HCX_FLAG_MINSTEP    0x40u  /* dt fell below dt_min */
 #define HCX_FLAG_SINGULAR   0x80u  /* zero pivot met in the final refinement */
 #define HCX_FLAG_ESCALATED  0x100u /* mixed-precision attempt failed; path re-tracked in FP64 */
+#define HCX_FLAG_REAL_REF   0x200u /* reference real test on the tracked root, before refinement:
+                                      tracked and all |Im x_i| <= real_ref_tol (Evaluations.cpp:88-110) */
+#define HCX_FLAG_UNIQUE_REF 0x400u /* counted unique by the reference's Find_Unique_Sols control flow
+                                      (Evaluations.cpp:127-176; set by hcx_collect) */
 
 #endif /* HCX_FLAGS_H */

@@ -47,12 +47,17 @@ class GPU_HC_Solver {
   void Data_Transfer_From_Host_To_Device();
   void Solve_by_GPU_HC();
 
-  // collection counters (Evaluations.hpp:47-50), over all instances
+  // collection counters (Evaluations.hpp:47-50), over all instances, with
+  // the reference's semantics (Evaluations.cpp:88-176): real = converged with
+  // every |Im x_i| <= 1e-4 on the tracked root; unique = Find_Unique_Sols
   unsigned Num_Of_Coverged_Sols = 0;
   unsigned Num_Of_Inf_Sols = 0;
   unsigned Num_Of_Real_Sols = 0;
   unsigned Num_Of_Unique_Sols = 0;
+  // root-level counts of the FP64-refined roots
   unsigned Num_Of_Refined_Sols = 0;
+  unsigned Num_Of_Refined_Real_Sols = 0;    // |Im x_i| <= 1e-8 on the refined root
+  unsigned Num_Of_Distinct_Sols = 0;        // first occurrence of each refined root
 
   // timers (seconds)
   double gpu_time = 0, transfer_h2d_time = 0, transfer_d2h_time = 0;

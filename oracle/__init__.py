@@ -14,6 +14,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 
 FLAG_TRACKED, FLAG_INFINITY, FLAG_REFINED, FLAG_REAL, FLAG_UNIQUE = 0x01, 0x02, 0x04, 0x08, 0x10
 FLAG_BUDGET, FLAG_MINSTEP, FLAG_SINGULAR = 0x20, 0x40, 0x80
+FLAG_REAL_REF, FLAG_UNIQUE_REF = 0x200, 0x400
 
 
 class Tables(C.Structure):
@@ -25,11 +26,11 @@ class Opts(C.Structure):
     _fields_ = [("max_steps", C.c_int), ("max_corr", C.c_int), ("incr_steps", C.c_int), ("dt0", C.c_double),
                 ("end_zone", C.c_double), ("t_tol", C.c_double), ("dt_min", C.c_double), ("corr_tol", C.c_double),
                 ("inf_tol", C.c_double), ("refine_iters", C.c_int), ("refine_tol", C.c_double),
-                ("real_tol", C.c_double)]
+                ("real_tol", C.c_double), ("real_ref_tol", C.c_double)]
 
 
 DEFAULTS = dict(max_steps=200, max_corr=3, incr_steps=3, dt0=0.05, end_zone=0.05, t_tol=1e-7, dt_min=1e-7,
-                corr_tol=1e-6, inf_tol=1e14, refine_iters=8, refine_tol=1e-10, real_tol=1e-8)
+                corr_tol=1e-6, inf_tol=1e14, refine_iters=8, refine_tol=1e-10, real_tol=1e-8, real_ref_tol=1e-4)
 
 _lib = None
 
@@ -153,6 +154,7 @@ def p2c_poly(scalar, params, p0, p1, M):
 def collect(n, I, K, roots, flags, tol=1e-4):
     roots = np.ascontiguousarray(roots, dtype=np.complex128)
     flags = np.ascontiguousarray(flags, dtype=np.uint32).copy()
-    counts = np.zeros(5, dtype=np.int64)
+    counts = np.zeros(7, dtype=np.int64)
     lib().orc_collect(n, I, K, _p(roots), _p(flags), tol, _p(counts))
-    return dict(zip(["tracked", "infinity", "refined", "real", "unique"], counts.tolist())), flags
+    return dict(zip(["tracked", "infinity", "refined", "real", "unique", "unique_ref", "real_ref"],
+                    counts.tolist())), flags

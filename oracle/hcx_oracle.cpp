@@ -12,6 +12,7 @@
 //   orc_track       HC_solver_<p> step loop           kernel_HC_Solver_generalized_3views_4pts.cu:96-264
 //   orc_p2c_affine  Construct_Coeffs_From_Params      Data_Reader.cpp:59-89 (affine closed form)
 //   orc_collect     Evaluate_GPUHC_Sols / Find_Unique_Sols  Evaluations.cpp:88-176
+//                   (F_REAL_REF is set by orc_track on the tracked root, :96-104)
 // Additions required by the tracker's contract (not in the reference): final
 // FP64 Newton refinement at t = 1, residual, real classification on refined
 // roots, optional minimum step.
@@ -45,6 +46,7 @@ typedef struct {
   double dt0, end_zone, t_tol, dt_min, corr_tol, inf_tol;
   int refine_iters;
   double refine_tol, real_tol;
+  double real_ref_tol;  // the reference's real test on the tracked root (definitions.hpp:25)
 } orc_opts;
 
 }  // extern "C"
@@ -52,7 +54,7 @@ typedef struct {
 namespace {
 
 const uint32_t F_TRACKED = 0x01, F_INF = 0x02, F_REFINED = 0x04, F_REAL = 0x08, F_UNIQUE = 0x10, F_BUDGET = 0x20,
-               F_MINSTEP = 0x40, F_SINGULAR = 0x80;
+               F_MINSTEP = 0x40, F_SINGULAR = 0x80, F_REAL_REF = 0x200, F_UNIQUE_REF = 0x400;
 
 // c_j(t) and dc_j/dt(t) from the coefficient-polynomial-in-t tables.
 void coeffs(const orc_tables& T, const cd* hx, const cd* ht, double t, std::vector<cd>& c, std::vector<cd>& cdot) {
@@ -145,7 +147,11 @@ int lu_solve(int n, cd* A, cd* b, cd* x, int* piv) {
     }
   }
   for (int i = n - 1; i >= 0; --i) {
-    const cd xi = b[i] / A[(size_t)i * n + i];
+    // MAGMA_C_DIV by a zero U_ii (a skipped zero pivot column) yields NaN in
+    // the reference (dev-cgesv-batched-small.cuh:97-106), not the infinity
+    // of C99 complex division
+    const cd uii = A[(size_t)i * n + i];
+    const cd xi = (uii == cd(0, 0)) ? cd(NAN, NAN) : b[i] / uii;
     x[i] = xi;
     for (int r = 0; r < i; ++r) b[r] -= A[(size_t)r * n + i] * xi;
   }
@@ -239,6 +245,13 @@ PathOut track_one(const orc_tables& T, const cd* hx, const cd* ht, const cd* sta
   else out.flags |= F_BUDGET;
   out.t = t;
   out.dt = dt;
+  // Evaluate_GPUHC_Sols's real test (Evaluations.cpp:96-104) on the tracked
+  // root: converged and every |Im x_i| <= ZERO_IMAG_PART_TOL_FOR_SP
+  if (out.flags & F_TRACKED) {
+    int real_vars = 0;
+    for (int i = 0; i < n; ++i) real_vars += std::fabs(x[i].imag()) <= o.real_ref_tol ? 1 : 0;
+    if (real_vars == n) out.flags |= F_REAL_REF;
+  }
   if ((out.flags & F_TRACKED) && o.refine_iters > 0) {
     bool conv = false, sing = false;
     for (int it = 0; it < o.refine_iters; ++it) {
@@ -421,33 +434,57 @@ int orc_track(const orc_tables* T, int K, const double* start, int I, const doub
   return 0;
 }
 
-// Counts: [tracked, infinity, refined, real, unique]; sets F_UNIQUE on first occurrences.
+// Counts: [tracked, infinity, refined, real, unique, unique_ref, real_ref].
+//  * unique: F_UNIQUE on the first occurrence of each refined root;
+//  * real_ref: tracked paths with F_REAL_REF (Evaluate_GPUHC_Sols, Evaluations.cpp:88-110);
+//  * unique_ref: Find_Unique_Sols (Evaluations.cpp:127-176) restated with its
+//    own sets: for each converged bs not in Skip, Duplicate = every later ds
+//    (converged or not) whose root agrees in every coordinate; an empty set
+//    counts bs (F_UNIQUE_REF), a non-empty set replaces Skip.  Duplicate is
+//    flushed only when Skip is non-empty, as in the reference.
 int orc_collect(int n, int I, int K, const double* roots, uint32_t* flags, double tol, long long* counts) {
-  for (int q = 0; q < 5; ++q) counts[q] = 0;
+  for (int q = 0; q < 7; ++q) counts[q] = 0;
   const cd* x = reinterpret_cast<const cd*>(roots);
+  auto same = [&](size_t a, size_t b) {
+    for (int v = 0; v < n; ++v) {
+      const cd p = x[a * n + v], q = x[b * n + v];
+      if (!(std::fabs(p.real() - q.real()) < tol && std::fabs(p.imag() - q.imag()) < tol)) return false;
+    }
+    return true;
+  };
   for (int inst = 0; inst < I; ++inst) {
     const size_t base = (size_t)inst * K;
     for (int j = 0; j < K; ++j) {
       uint32_t& f = flags[base + j];
-      f &= ~F_UNIQUE;
+      f &= ~(F_UNIQUE | F_UNIQUE_REF);
       if (f & F_TRACKED) counts[0]++;
       if (f & F_INF) counts[1]++;
+      if ((f & F_TRACKED) && (f & F_REAL_REF)) counts[6]++;
       if (!(f & F_REFINED)) continue;
       counts[2]++;
       if (f & F_REAL) counts[3]++;
       bool dup = false;
-      for (int i = 0; i < j && !dup; ++i) {
-        if (!(flags[base + i] & F_REFINED)) continue;
-        bool same = true;
-        for (int v = 0; v < n && same; ++v) {
-          const cd a = x[(base + i) * n + v], b = x[(base + j) * n + v];
-          same = std::fabs(a.real() - b.real()) < tol && std::fabs(a.imag() - b.imag()) < tol;
-        }
-        dup = same;
-      }
+      for (int i = 0; i < j && !dup; ++i) dup = (flags[base + i] & F_REFINED) && same(base + i, base + j);
       if (!dup) {
         f |= F_UNIQUE;
         counts[4]++;
+      }
+    }
+    // Find_Unique_Sols, literally: std::set Skip / Duplicate
+    std::vector<int> skip, dupl;
+    for (int bs = 0; bs < K; ++bs) {
+      if (!(flags[base + bs] & F_TRACKED)) continue;
+      if (!skip.empty()) {
+        if (std::find(skip.begin(), skip.end(), bs) != skip.end()) continue;
+        dupl.clear();
+      }
+      for (int ds = bs + 1; ds < K; ++ds)
+        if (same(base + bs, base + ds) && std::find(dupl.begin(), dupl.end(), ds) == dupl.end()) dupl.push_back(ds);
+      if (dupl.empty()) {
+        flags[base + bs] |= F_UNIQUE_REF;
+        counts[5]++;
+      } else {
+        skip = dupl;
       }
     }
   }

@@ -42,6 +42,7 @@ def test_default_options_are_reference_constants(hcx_mod):
     assert o.corr_tol == 1e-6 and o.inf_tol == 1e14
     assert o.unique_tol == 1e-4  # definitions.hpp:24
     assert o.real_tol == 1e-8  # definitions.hpp:26 (DP)
+    assert o.real_ref_tol == 1e-4  # definitions.hpp:25 (SP), the reference's own real test
 
 
 def test_no_device_fails_loudly_without_gpu(hcx_mod):

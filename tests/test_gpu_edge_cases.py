@@ -38,7 +38,7 @@ def test_empty_batch(hcx_mod, gpu):
         torch.cuda.synchronize()
         c = hcx.collect(r, opts)
         print(f"[empty prec={prec}] {c}", flush=True)
-        assert c == {"tracked": 0, "infinity": 0, "refined": 0, "real": 0, "unique": 0}
+        assert c == {"tracked": 0, "infinity": 0, "refined": 0, "real": 0, "unique": 0, "unique_ref": 0, "real_ref": 0}
     # the problem is still usable afterwards
     b1 = hcx.Batch(prob, 1, target_params=workloads.targets(ty["k"], 1, seed=3))
     g = hcx.track(prob, b1, hcx.default_opts()).to_host()

@@ -86,7 +86,7 @@ def parse_summary(text):
     vals = {}
     for line in text.splitlines():
         for key in ("converged (t = 1)", "refined in FP64", "infinity failures", "real solutions",
-                    "unique solutions"):
+                    "unique solutions", "real refined roots", "distinct refined roots"):
             if line.startswith("- " + key):
                 vals[key] = int(line.split(":")[-1])
     return vals
@@ -116,6 +116,18 @@ def test_cli_solves_reference_folder(tmp_path, name):
         slack = 12 if unstable_ok else 0
         assert abs(v["refined in FP64"] - counts["refined"]) <= slack, (v, counts)
         assert abs(v["converged (t = 1)"] - counts["tracked"]) <= slack
-        assert abs(v["real solutions"] - counts["real"]) <= slack
+        # the reference's own counts (tracked-root real test, Find_Unique_Sols) and the root-level ones
+        assert abs(v["real solutions"] - counts["real_ref"]) <= slack
+        assert abs(v["real refined roots"] - counts["real"]) <= slack
+        if unstable_ok:
+            # with up to 100 corrector iterations per step, paths of this
+            # problem jump between basins under rounding-level changes, so
+            # which roots coincide differs between runs; the counts are
+            # pinned exactly on equal inputs by test_device_collect_equals_oracle
+            assert 0 <= v["unique solutions"] <= v["converged (t = 1)"]
+            assert 0 < v["distinct refined roots"] <= v["refined in FP64"]
+        else:
+            assert v["unique solutions"] == counts["unique_ref"]
+            assert v["distinct refined roots"] == counts["unique"]
         text = open(str(tmp_path / "GPU_Converged_HC_tracks.txt")).read()
         assert text.count("\n\n") >= v["refined in FP64"] - 1
