@@ -604,6 +604,46 @@ LaunchWork seg_work(const WsBlock& b, int s, long long inst_off, long long path_
   return w;
 }
 
+// ------------------------------------------------------------ static scratch
+// Launch workspace of hcx_track_noalloc: a file-scope device array (each GPU
+// holds its own copy until the process ends).  Host code never uses the
+// array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+constexpr size_t kScratchBytes =
+    ws_layout(sizeof(SegOut), HCX_NOALLOC_MAX_INSTANCES, HCX_NOALLOC_MAX_PATHS).bytes;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's scratch as a workspace block.
+hcx_status scratch_block(WsBlock& b) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("hcx_track_noalloc: device %d beyond the %d devices the scratch table covers", dev, kMaxDevices);
+    return HCX_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hcx_track_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+      return HCX_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  b.base = g_scratch_ptr[dev];
+  b.inst_cap = HCX_NOALLOC_MAX_INSTANCES;
+  b.paths_cap = HCX_NOALLOC_MAX_PATHS;
+  b.pinned = true;
+  return HCX_OK;
+}
+}  // namespace
+
 // A launch on `stream` with that stream's pooled workspace.
 hcx_status track_pooled(int nseg, hcx_problem* const* probs, const hcx_batch_desc* batches,
                         const hcx_result_desc* results, const hcx_track_opts* o, hipStream_t stream, int persistent,
@@ -1271,6 +1311,31 @@ double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, 
     return -1.0;
   }
   return ms * 1e-3;
+}
+
+// hcx_track with the static scratch as its launch workspace: nothing is
+// allocated (no hipMalloc, no pool, no allocating library call).
+hcx_status hcx_track_noalloc(hcx_problem* p, const hcx_batch_desc* b, const hcx_track_opts* o,
+                             const hcx_result_desc* r, void* stream) {
+  clear_error();
+  if (!p || !b || !r || !o) {
+    set_error("hcx_track_noalloc: null argument");
+    return HCX_ERR_INVALID_ARG;
+  }
+  long long inst = 0, paths = 0;
+  ws_need(1, &p, b, r, o, inst, paths);
+  if (inst > HCX_NOALLOC_MAX_INSTANCES || paths > HCX_NOALLOC_MAX_PATHS) {
+    set_error("hcx_track_noalloc: the batch needs %lld instance counters and %lld re-track entries; the static "
+              "scratch holds %d and %d", inst, paths, HCX_NOALLOC_MAX_INSTANCES, HCX_NOALLOC_MAX_PATHS);
+    return HCX_ERR_INVALID_ARG;
+  }
+  WsBlock blk;
+  hcx_status st = scratch_block(blk);
+  if (st != HCX_OK) return st;
+  LaunchWork work[1];
+  ws_carve(blk, 1, b, r, &p, o, work);
+  return track_impl(1, &p, b, r, o, (hipStream_t)stream, 1, reinterpret_cast<unsigned long long*>(r->d_batch_start),
+                    work);
 }
 
 // Reference-style per-instance launch with the generic result layout (no

@@ -46,15 +46,15 @@ namespace hcx {
 constexpr int kSegSlots = 8;
 constexpr size_t kCounterBytes = (size_t)kSegSlots * 64 * sizeof(uint32_t);
 
-inline size_t ws_align(size_t n) { return (n + 255) & ~size_t(255); }
+constexpr size_t ws_align(size_t n) { return (n + 255) & ~size_t(255); }
 
 // Byte layout of a block holding `inst` instance counters and `paths`
 // re-track list entries (both int32) after the fixed part.
 struct WsLayout {
   size_t out_off, inst_off, plist_off, bytes;
 };
-inline WsLayout ws_layout(size_t segout_bytes, long long inst, long long paths) {
-  WsLayout L;
+constexpr WsLayout ws_layout(size_t segout_bytes, long long inst, long long paths) {
+  WsLayout L{};
   L.out_off = kCounterBytes;
   L.inst_off = ws_align(L.out_off + (size_t)kSegSlots * segout_bytes);
   L.plist_off = ws_align(L.inst_off + sizeof(int32_t) * (size_t)(inst > 0 ? inst : 0));

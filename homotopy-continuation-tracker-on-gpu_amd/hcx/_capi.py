@@ -120,6 +120,8 @@ SIGNATURES = {
                             C.c_void_p]),
     "hcx_track_static": (C.c_int, [C.c_void_p, C.POINTER(BatchDesc), C.POINTER(TrackOpts), C.POINTER(ResultDesc),
                                    C.c_void_p]),
+    "hcx_track_noalloc": (C.c_int, [C.c_void_p, C.POINTER(BatchDesc), C.POINTER(TrackOpts), C.POINTER(ResultDesc),
+                                    C.c_void_p]),
     "hcx_track_multi": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(BatchDesc), C.POINTER(ResultDesc),
                                   C.POINTER(TrackOpts), C.c_void_p]),
     "hcx_kernel_HC_Solver": (C.c_double, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,

@@ -196,6 +196,23 @@ typedef struct {
 hcx_status hcx_track(hcx_problem* p, const hcx_batch_desc* b, const hcx_track_opts* o,
                      const hcx_result_desc* r, void* stream);
 
+/* hcx_track that allocates nothing.  No counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Inputs and outputs
+ * are the caller's as for hcx_track (a batch of target parameters still
+ * brings its d_coeff_work); the launch workspace is a static device array of
+ * about 4.3 MiB that each GPU holds until the process ends, with room for
+ * HCX_NOALLOC_MAX_INSTANCES instance counters (used when d_inst_done is set)
+ * and HCX_NOALLOC_MAX_PATHS re-track entries (used in mixed precision); a
+ * larger batch is rejected with HCX_ERR_INVALID_ARG.  Every call on one GPU
+ * shares that array and it keeps the last call's contents (the launch
+ * clears what it needs): calls that overlap on one GPU (several streams or
+ * threads) race. */
+#define HCX_NOALLOC_MAX_INSTANCES 65536
+#define HCX_NOALLOC_MAX_PATHS 1048576
+hcx_status hcx_track_noalloc(hcx_problem* p, const hcx_batch_desc* b, const hcx_track_opts* o,
+                             const hcx_result_desc* r, void* stream);
+
 /* Frees the launch workspace of `stream` after waiting for the stream (call
  * before destroying a stream that tracked; a workspace that a captured graph
  * refers to stays alive).  No counterpart in the reference, whose launcher
