@@ -163,9 +163,10 @@ def main():
     ap.add_argument("--inflight", type=int, default=3,
                     help="configs 1, 2: solves in flight on separate streams (the duplicate re-track "
                          "and tail of one solve overlap the next ones)")
-    ap.add_argument("--waves-per-cu", type=int, default=7,
-                    help="config 2: persistent grid density (waves per CU); one below the n = 6 "
-                         "kernel's occupancy leaves slots for the previous solve's re-track and tail")
+    ap.add_argument("--waves-per-cu", type=int, default=-1,
+                    help="config 2: persistent grid density (waves per CU; < 0: occupancy less that many); "
+                         "one below the kernel's occupancy leaves slots for the previous solve's re-track "
+                         "and tail")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="default run: seconds of each serving window (fixed rate, then the offered 2000 req/s)")
     ap.add_argument("--serve-rate", type=float, default=800.0,

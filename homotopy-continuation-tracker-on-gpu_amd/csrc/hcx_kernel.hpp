@@ -273,21 +273,33 @@ __device__ __forceinline__ void group_argmax(T& v, int& key, const Group& g) {
   }
 }
 
-// ------------------------------------------------------------ group LU
+// ------------------------------------------------------------ group solve
 // Solves A x = b for the group's n x n system, row r held by lane r (A[j] =
-// A[r][j]).  Returns x_gl to lane gl.  Partial pivoting on cabs1 with the
-// first maximum in current row order (rows carry a position label that is
-// swapped like a physical row interchange); a zero pivot column is skipped
-// and reported through `singular` (dev-cgesv-batched-small.cuh:51-95).
+// A[r][j]).  Returns x_gl to lane gl.  Pivoting is the reference's partial
+// pivoting (dev-cgesv-batched-small.cuh:51-95): cabs1, the first maximum in
+// current row order (rows carry a position label swapped like a physical
+// row interchange), multipliers by the pivot reciprocal, a zero pivot column
+// skipped and reported through `singular`.  Every step eliminates the pivot
+// column from the rows ABOVE the pivot as well (Gauss-Jordan), in place of the
+// reference's back substitution (:97-106): on a lane group the lanes of those
+// rows would only wait during the step, so the extra row updates cost no
+// issue cycles, while the n-step dependent chain of the back substitution
+// (a lane exchange per step) and the need to keep U alive for it disappear
+// (n = 30 FP64: 157 instead of about 290 registers).  The candidate rows of
+// each pivot search receive exactly the updates of the LU, so the pivot
+// sequence is the LU's.  At the end the lane holding position i has
+// x_i = b_i / U_ii; one LDS exchange returns x_gl to lane gl.  A zero pivot
+// makes every component NaN (the reference's division by a zero U_ii gives
+// NaN in that component and in the rows that depend on it; the tracker only
+// reacts to whether any component is not finite).
 template <int NMAX, int GC, class T>
 __device__ __forceinline__ cx<T> group_lu_solve(cx<T> (&A)[NMAX], cx<T> b, const Group& g, int n, bool row,
                                                 cx<T>* sp, bool& singular) {
   int label = g.gl;
   cx<T> own_rinv = cmk<T>(T(0), T(0));  // 1 / U_ii of the row this lane ends up holding
-  uint32_t pl[(NMAX + 3) / 4];
-#pragma unroll
-  for (int q = 0; q < (NMAX + 3) / 4; ++q) pl[q] = 0u;
   singular = false;
+  // sp shares the group's LDS slots with x: the evaluation's reads of x come first
+  HCX_WAVE_SYNC();
 #pragma unroll
   for (int k = 0; k < NMAX; ++k) {
     if (k < n) {
@@ -298,21 +310,19 @@ __device__ __forceinline__ cx<T> group_lu_solve(cx<T> (&A)[NMAX], cx<T> b, const
       const int bl = key & 63, blabel = key >> 6;
       const bool zero = !(v > T(0));
       singular = singular || zero;
-      if (g.in_grp && g.gl == bl) {
+      const bool piv = g.gl == bl;
+      if (g.in_grp && piv) {
 #pragma unroll
         for (int j = k; j < NMAX; ++j)
           if (j < n) sp[j] = A[j];
         sp[NMAX] = b;
       }
       HCX_WAVE_SYNC();
-      if (g.gl == bl) label = k;
+      if (piv) label = k;
       else if (label == k) label = blabel;
-      pl[k >> 2] |= (uint32_t)bl << (8 * (k & 3));
       const cx<T> rinv = zero ? cmk<T>(T(1), T(0)) : crecip(sp[k]);
-      // a zero pivot makes the back substitution produce NaN, as the
-      // reference's division by a zero U_ii does
-      if (g.gl == bl) own_rinv = zero ? cmk<T>(T(NAN), T(NAN)) : rinv;
-      if (row && label > k && !zero) {
+      if (piv) own_rinv = rinv;
+      if (row && !piv && !zero) {
         const cx<T> l = cmul(A[k], rinv);
 #pragma unroll
         for (int j = k + 1; j < NMAX; ++j)
@@ -322,30 +332,25 @@ __device__ __forceinline__ cx<T> group_lu_solve(cx<T> (&A)[NMAX], cx<T> b, const
       HCX_WAVE_SYNC();
     }
   }
-  // back substitution (dev-cgesv-batched-small.cuh:97-106): x_i from the
-  // lane holding row position i (times its stored 1/U_ii), broadcast, then
-  // eliminated from the rows above
-  cx<T> sol = cmk<T>(T(0), T(0));
-#pragma unroll
-  for (int i = NMAX - 1; i >= 0; --i) {
-    if (i < n) {
-      const int src = g.gbase + (int)((pl[i >> 2] >> (8 * (i & 3))) & 0xffu);
-      const cx<T> q = cmul(b, own_rinv);
-      const cx<T> xi = lane_read(q, src);
-      if (row && label < i) b = cfnms(b, A[i], xi);
-      if (g.gl == i) sol = xi;
-    }
-  }
+  const cx<T> xi = cmul(b, own_rinv);
+  if (row) sp[label] = xi;
+  HCX_WAVE_SYNC();
+  cx<T> sol = row ? sp[g.gl] : cmk<T>(T(0), T(0));
+  HCX_WAVE_SYNC();
+  if (singular) sol = cmk<T>(T(NAN), T(NAN));
   return sol;
 }
 
-// Per-group view of the LDS region.
+// Per-group view of the LDS region.  x (read by the evaluation) and the
+// pivot-row buffer (written by the LU that follows it) are never live at the
+// same time, so they share the first NMAX + 1 slots; group_lu_solve starts
+// with a wave-scope fence that keeps the evaluation's reads before its writes.
 template <int NMAX>
 struct Lds {
   unsigned char* base;
   __device__ cx<double>* x64() const { return reinterpret_cast<cx<double>*>(base); }
-  __device__ cx<double>* piv64() const { return x64() + (NMAX + 1); }
-  __device__ unsigned char* coef() const { return reinterpret_cast<unsigned char*>(piv64() + (NMAX + 1)); }
+  __device__ cx<double>* piv64() const { return x64(); }
+  __device__ unsigned char* coef() const { return reinterpret_cast<unsigned char*>(x64() + (NMAX + 1)); }
   template <class T>
   __device__ cx<T>* x() const { return reinterpret_cast<cx<T>*>(base); }
   template <class T>
@@ -431,18 +436,19 @@ __device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, cons
 // predictor (Jacobian, dH/dt and LU of the four stages), FP64 Newton
 // corrector (FP32 before the end zone when o.newton32), FP64 refinement; a
 // path that ends without a refined root is re-tracked from t = 0 entirely in
-// FP64 (escalation): by the same group (KESC), or, for the size-specialised
-// instances n = 9..16, by the FP64 pass after the launch, which keeps the FP64
-// predictor out of the kernel: its register footprint then allows one more
-// wave per SIMD (n = 12: 260 -> 241 VGPRs).  The pass skips paths already
-// escalated, so either way each failed path is re-tracked once.
+// FP64 (escalation): by the same group (KESC: the generic instances and
+// n = 30), or, for the size-specialised instances up to n = 16, by the FP64
+// pass after the launch, which keeps the FP64 predictor out of the kernel:
+// its register footprint then allows one more wave per SIMD (n = 6: 214 ->
+// 166 VGPRs, n = 12: 260 -> 241).  The pass skips paths already escalated,
+// so either way each failed path is re-tracked once.
 template <int NMAX, int DEG, int MODE, bool EXACT>
 __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned char* smem, int lane) {
   // local copies: the fields are wave-uniform and live in SGPRs (scalar
   // kernarg loads) instead of being re-read through a generic pointer
   Seg S = LN.seg[si];
   constexpr int GE = exact_lanes(NMAX);
-  constexpr bool KESC = !(EXACT && NMAX >= 9 && NMAX <= 16);
+  constexpr bool KESC = !(EXACT && NMAX <= 16);
   if (EXACT) {
     // size-specialised instance: n = NMAX and G are compile-time constants,
     // so every loop bound and the lane-group geometry fold away

@@ -173,44 +173,50 @@ __device__ __forceinline__ bool same_root(const double2* a, const double2* b, in
 //  * unique_ref: Find_Unique_Sols (:127-176).  Its Skip set is always the
 //    Duplicate set of the last examined path that had one (bstar), so the
 //    sequential pass needs, per path, only "has a later path (any outcome)
-//    with the same root" -- computed in parallel into an LDS bit set -- and a
-//    comparison with bstar.  Dynamic LDS: 3 * ceil(K / 32) words.
+//    with the same root" and a comparison with bstar.
+// The pair comparisons run in parallel over the block (thread per first
+// index, roots staged in LDS when they fit) into LDS bit sets; one thread
+// then runs the short sequential pass.  Dynamic LDS: 4 bit sets of
+// ceil(K / 32) words, then K * n roots when stage_roots is set.
 __global__ void collect_kernel(int n, int K, const double2* __restrict__ roots, uint32_t* __restrict__ flags,
-                               double tol, unsigned long long* __restrict__ counts) {
-  extern __shared__ unsigned int bits[];
+                               double tol, unsigned long long* __restrict__ counts, int stage_roots) {
+  extern __shared__ __align__(16) unsigned int bits[];
   const int words = (K + 31) / 32;
-  unsigned int* later = bits;               // a later path has the same root
-  unsigned int* tracked = bits + words;     // tracked ("converged")
-  unsigned int* uref = bits + 2 * words;    // counted by the unique_ref pass
+  unsigned int* later = bits;                // a later path (any outcome) has the same root
+  unsigned int* tracked = bits + words;      // tracked ("converged")
+  unsigned int* dupref = bits + 2 * words;   // refined with an earlier refined duplicate
+  unsigned int* uref = bits + 3 * words;     // counted by the unique_ref pass
+  double2* sroots = reinterpret_cast<double2*>(bits + ((4 * words + 3) & ~3));
   const int inst = blockIdx.x;
   const size_t base = (size_t)inst * K;
-  for (int w = threadIdx.x; w < 3 * words; w += blockDim.x) bits[w] = 0u;
+  for (int w = threadIdx.x; w < 4 * words; w += blockDim.x) bits[w] = 0u;
+  const double2* x = roots + base * n;
+  if (stage_roots) {
+    for (int q = threadIdx.x; q < K * n; q += blockDim.x) sroots[q] = x[q];
+    x = sroots;
+  }
   __syncthreads();
-  unsigned long long c_tr = 0, c_inf = 0, c_ref = 0, c_real = 0, c_uni = 0, c_rref = 0;
-  for (int j = threadIdx.x; j < K; j += blockDim.x) {
-    uint32_t f = flags[base + j] & ~(HCX_FLAG_UNIQUE | HCX_FLAG_UNIQUE_REF);
-    const double2* xj = roots + (base + j) * n;
-    if (f & HCX_FLAG_TRACKED) {
+  unsigned long long c_tr = 0, c_inf = 0, c_ref = 0, c_real = 0, c_rref = 0;
+  for (int i = threadIdx.x; i < K; i += blockDim.x) {
+    const uint32_t fi = flags[base + i];
+    if (fi & HCX_FLAG_TRACKED) {
       ++c_tr;
-      atomicOr(&tracked[j >> 5], 1u << (j & 31));
-      if (f & HCX_FLAG_REAL_REF) ++c_rref;
+      atomicOr(&tracked[i >> 5], 1u << (i & 31));
+      if (fi & HCX_FLAG_REAL_REF) ++c_rref;
     }
-    c_inf += (f & HCX_FLAG_INFINITY) ? 1 : 0;
-    if (f & HCX_FLAG_REFINED) {
+    c_inf += (fi & HCX_FLAG_INFINITY) ? 1 : 0;
+    const bool ri = (fi & HCX_FLAG_REFINED) != 0;
+    if (ri) {
       ++c_ref;
-      c_real += (f & HCX_FLAG_REAL) ? 1 : 0;
-      bool dup = false;
-      for (int i = 0; i < j && !dup; ++i)
-        dup = (flags[base + i] & HCX_FLAG_REFINED) && same_root(roots + (base + i) * n, xj, n, tol);
-      if (!dup) {
-        f |= HCX_FLAG_UNIQUE;
-        ++c_uni;
-      }
+      c_real += (fi & HCX_FLAG_REAL) ? 1 : 0;
     }
     bool lat = false;
-    for (int i = j + 1; i < K && !lat; ++i) lat = same_root(roots + (base + i) * n, xj, n, tol);
-    if (lat) atomicOr(&later[j >> 5], 1u << (j & 31));
-    flags[base + j] = f;
+    for (int j = i + 1; j < K; ++j) {
+      if (!same_root(x + (size_t)i * n, x + (size_t)j * n, n, tol)) continue;
+      lat = true;
+      if (ri && (flags[base + j] & HCX_FLAG_REFINED)) atomicOr(&dupref[j >> 5], 1u << (j & 31));
+    }
+    if (lat) atomicOr(&later[i >> 5], 1u << (i & 31));
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -218,7 +224,7 @@ __global__ void collect_kernel(int n, int K, const double2* __restrict__ roots, 
     int bstar = -1;
     for (int bs = 0; bs < K; ++bs) {
       if (!((tracked[bs >> 5] >> (bs & 31)) & 1u)) continue;
-      if (bstar >= 0 && same_root(roots + (base + bs) * n, roots + (base + bstar) * n, n, tol)) continue;
+      if (bstar >= 0 && same_root(x + (size_t)bs * n, x + (size_t)bstar * n, n, tol)) continue;
       if ((later[bs >> 5] >> (bs & 31)) & 1u) {
         bstar = bs;
       } else {
@@ -229,8 +235,16 @@ __global__ void collect_kernel(int n, int K, const double2* __restrict__ roots, 
     atomicAdd(&counts[5], c_uref);
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < K; j += blockDim.x)
-    if ((uref[j >> 5] >> (j & 31)) & 1u) flags[base + j] |= HCX_FLAG_UNIQUE_REF;
+  unsigned long long c_uni = 0;
+  for (int j = threadIdx.x; j < K; j += blockDim.x) {
+    uint32_t f = flags[base + j] & ~(HCX_FLAG_UNIQUE | HCX_FLAG_UNIQUE_REF);
+    if ((f & HCX_FLAG_REFINED) && !((dupref[j >> 5] >> (j & 31)) & 1u)) {
+      f |= HCX_FLAG_UNIQUE;
+      ++c_uni;
+    }
+    if ((uref[j >> 5] >> (j & 31)) & 1u) f |= HCX_FLAG_UNIQUE_REF;
+    flags[base + j] = f;
+  }
   atomicAdd(&counts[0], c_tr);
   atomicAdd(&counts[1], c_inf);
   atomicAdd(&counts[2], c_ref);
@@ -409,7 +423,8 @@ hcx_status validate_opts(const hcx_track_opts* o) {
   }
   if (o->max_steps < 0 || o->max_corr < 1 || o->incr_steps < 1 || !(o->dt0 > 0) || !(o->end_zone >= 0) ||
       !(o->t_tol > 0) || o->refine_iters < 0 || (o->precision != HCX_PREC_FP64 && o->precision != HCX_PREC_MIXED) ||
-      o->lanes_per_path < 0 || o->lanes_per_path > 64 || o->specialize < 0 || o->specialize > 1) {
+      o->lanes_per_path < 0 || o->lanes_per_path > 64 || o->specialize < 0 || o->specialize > 1 ||
+      o->waves_per_cu < -64 || o->waves_per_cu > 64) {
     set_error("invalid tracker options (max_steps=%d max_corr=%d incr=%d precision=%d lanes=%d)", o->max_steps,
               o->max_corr, o->incr_steps, o->precision, o->lanes_per_path);
     return HCX_ERR_INVALID_ARG;
@@ -437,17 +452,17 @@ Opts to_opts(const hcx_track_opts* o) {
   return r;
 }
 
-// Per-group LDS: x and pivot-row buffers (FP64 capacity) and the coefficient
-// area: with M = 1 (dc/dt constant, read by the FP64 stages from the
-// instance's table in global memory) FP32 c + dc/dt overlaid by FP64 c, else
-// (or dot_lds: FP64 stages with dc/dt in LDS) FP64 c + dc/dt.
+// Per-group LDS: the x / pivot-row slots (shared, FP64 capacity) and the
+// coefficient area: with M = 1 (dc/dt constant, read by the FP64 stages from
+// the instance's table in global memory) FP32 c + dc/dt overlaid by FP64 c,
+// else (or dot_lds: FP64 stages with dc/dt in LDS) FP64 c + dc/dt.
 // The region stride is padded to 16 mod 256 bytes: the groups of a wave read
 // the same offsets of their regions together (pivot-row broadcasts, x and
 // coefficient gathers), and this stride spreads them over distinct LDS banks
 // (64 banks x 4 bytes) for 8- and 16-byte accesses.
 int lds_group_bytes(int nmax, int C, int M, bool dot_lds = false) {
   const int per = (M > 1 || dot_lds) ? 32 : 16;
-  int b = 2 * 16 * (nmax + 1) + (C + 1) * per;
+  int b = 16 * (nmax + 1) + (C + 1) * per;
   b = (b + 15) & ~15;
   return ((b - 16 + 255) / 256) * 256 + 16;
 }
@@ -1111,11 +1126,13 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
   long long blocks;
   if (persistent) {
     int per_cu = o->waves_per_cu;
-    if (per_cu <= 0 && all_tuned && nseg == 1) per_cu = probs[0]->tuned_wpc;
+    if (per_cu == 0 && all_tuned && nseg == 1) per_cu = probs[0]->tuned_wpc;
     if (per_cu <= 0) {
+      // occupancy, less -waves_per_cu waves when that is negative (room for
+      // the work of other solves in flight on other streams)
       int nb = 0;
       HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(pl.fn), 64, pl.lds));
-      per_cu = std::max(nb, 1);
+      per_cu = std::max(nb + std::min(per_cu, 0), 1);
     }
     blocks = std::min<long long>((long long)per_cu * g_cu, std::max<long long>(total_groups_needed, 1));
     // proportional split by estimated work (paths * n^2 as a cost proxy)
@@ -1359,11 +1376,15 @@ hcx_status hcx_collect(int n, int num_instances, int num_tracks, const hcx_compl
     set_error("hcx_collect: invalid arguments");
     return HCX_ERR_INVALID_ARG;
   }
-  const size_t bit_lds = 3 * sizeof(unsigned int) * (size_t)((num_tracks + 31) / 32);
+  // bit sets, then the instance's roots when they fit in 48 KiB of LDS
+  const size_t bit_lds = 16 * (size_t)((4 * ((num_tracks + 31) / 32) + 3) / 4);
+  const size_t root_lds = sizeof(double2) * (size_t)num_tracks * n;
   if (bit_lds > 65536) {
     set_error("hcx_collect: %d paths per instance exceed the collection kernel's bit sets", num_tracks);
     return HCX_ERR_UNSUPPORTED;
   }
+  const int stage = bit_lds + root_lds <= 48 * 1024 ? 1 : 0;
+  const size_t lds = bit_lds + (stage ? root_lds : 0);
   hcx_status st = ensure_device_info();
   if (st != HCX_OK) return st;
   std::lock_guard<std::mutex> lk(g_mu);
@@ -1371,8 +1392,8 @@ hcx_status hcx_collect(int n, int num_instances, int num_tracks, const hcx_compl
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, g_counts, kNumCounts);
   if (!empty)
-    hipLaunchKernelGGL(collect_kernel, dim3(num_instances), dim3(256), bit_lds, s, n, num_tracks,
-                       reinterpret_cast<const double2*>(d_roots), d_flags, unique_tol, g_counts);
+    hipLaunchKernelGGL(collect_kernel, dim3(num_instances), dim3(256), lds, s, n, num_tracks,
+                       reinterpret_cast<const double2*>(d_roots), d_flags, unique_tol, g_counts, stage);
   HIP_TRY(hipGetLastError());
   unsigned long long h[kNumCounts];
   HIP_TRY(hipMemcpyAsync(h, g_counts, sizeof(h), hipMemcpyDeviceToHost, s));

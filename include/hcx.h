@@ -75,7 +75,8 @@ typedef struct {
   double unique_tol;      /* duplicate if all |dRe|,|dIm| < unique_tol (definitions.hpp:24) */
   int precision;          /* hcx_precision */
   int lanes_per_path;     /* lanes of a wave64 per path (G); 0 = autotuned / n */
-  int waves_per_cu;       /* persistent grid size per CU; 0 = from occupancy */
+  int waves_per_cu;       /* persistent grid size per CU; 0 = from occupancy; < 0: occupancy less that
+                             many waves (room for other solves in flight on other streams) */
   int specialize;         /* 1: size-specialised kernel instance when one exists (n = G); 0: generic */
   int retrack_duplicates; /* mixed precision: re-track in FP64 every path whose refined root duplicates
                              another root of its instance (path jumping), so roots and counts are the
