@@ -15,6 +15,12 @@ using dev::cmk;
 
 constexpr int kMaxSeg = 8;
 constexpr int kMaxN = 32;
+// terms whose table records are requested together when the tables are read
+// from global memory
+#ifndef HCX_TERM_CHUNK
+#define HCX_TERM_CHUNK 4
+#endif
+constexpr int kTermChunk = HCX_TERM_CHUNK;
 
 // Lanes per path of the size-specialised instance for n: n, except n = 30,
 // which runs on aligned 32-lane groups (still two paths per wave) so that
@@ -121,14 +127,23 @@ __device__ __forceinline__ void eval_jac_row(const Seg& S, int gle, const cx<T>*
           acc = cadd(acc, v);
         }
       } else {
+        // tables in global memory: the records of a chunk of terms are all
+        // requested before the first is used, so their latencies overlap
         const Term* tp = S.hx + (size_t)j * S.hxT * n + gle;
-#pragma unroll 2
-        for (int k = 0; k < cnt; ++k) {
-          const Term tm = tp[(size_t)k * n];
-          cx<T> v = cscale(sc[tm.c], (T)tm.s);
+        for (int k0 = 0; k0 < cnt; k0 += kTermChunk) {
+          Term tm[kTermChunk];
 #pragma unroll
-          for (int q = 0; q < DEG - 1; ++q) v = cmul(v, sx[term_var(tm, q)]);
-          acc = cadd(acc, v);
+          for (int q = 0; q < kTermChunk; ++q)
+            if (k0 + q < cnt) tm[q] = tp[(size_t)(k0 + q) * n];
+#pragma unroll
+          for (int q = 0; q < kTermChunk; ++q) {
+            if (k0 + q < cnt) {
+              cx<T> v = cscale(sc[tm[q].c], (T)tm[q].s);
+#pragma unroll
+              for (int d = 0; d < DEG - 1; ++d) v = cmul(v, sx[term_var(tm[q], d)]);
+              acc = cadd(acc, v);
+            }
+          }
         }
       }
     }
@@ -157,14 +172,27 @@ __device__ __forceinline__ cx<T> eval_t_row(const Seg& S, int gle, const cx<T>* 
       acc = cadd(acc, v);
     }
   } else {
+    // records (and, with remap, the coefficient numbers) of a chunk of terms
+    // are requested together, as in eval_jac_row
     const Term* tp = S.ht + gle;
-#pragma unroll 2
-    for (int k = 0; k < S.htT; ++k) {
-      const Term tm = tp[(size_t)k * n];
-      cx<T> v = cscale(coef[remap ? remap[tm.c] : tm.c], (T)tm.s);
+    for (int k0 = 0; k0 < S.htT; k0 += kTermChunk) {
+      Term tm[kTermChunk];
+      int ci[kTermChunk];
 #pragma unroll
-      for (int q = 0; q < DEG; ++q) v = cmul(v, sx[term_var(tm, q)]);
-      acc = cadd(acc, v);
+      for (int q = 0; q < kTermChunk; ++q)
+        if (k0 + q < S.htT) tm[q] = tp[(size_t)(k0 + q) * n];
+#pragma unroll
+      for (int q = 0; q < kTermChunk; ++q)
+        if (k0 + q < S.htT) ci[q] = remap ? remap[tm[q].c] : tm[q].c;
+#pragma unroll
+      for (int q = 0; q < kTermChunk; ++q) {
+        if (k0 + q < S.htT) {
+          cx<T> v = cscale(coef[ci[q]], (T)tm[q].s);
+#pragma unroll
+          for (int d = 0; d < DEG; ++d) v = cmul(v, sx[term_var(tm[q], d)]);
+          acc = cadd(acc, v);
+        }
+      }
     }
   }
   return acc;
