@@ -644,6 +644,11 @@ def _serve(srv, tids, types, rate, duration, seed):
             "submit_seconds": t_sub, "total_seconds": t_end, "max_submit_lateness_ms": late * 1e3, "stats": st}
 
 
+def _prec(args):
+    import hcx
+    return hcx.PREC_MIXED if args.precision == "mixed" else hcx.PREC_FP64
+
+
 def serving_leg(args):
     """Bounded serving windows for the default run: Poisson arrivals over the
     six resident types n in {3, 6, 9, 12, 16, 30} at a fixed rate (the stated
@@ -653,7 +658,7 @@ def serving_leg(args):
     log("bench: serving leg, seeding start roots of six types")
     types = workloads.quadratic_types(workloads.CONFIG5_TYPES, seed=args.seed)
     probs = [hcx.Problem.from_system(t["system"], t["start"], p0=t["p0"]) for t in types]
-    srv = hcx.Server(hcx.default_opts(precision=hcx.PREC_MIXED), hbm_budget_bytes=4 << 30, max_batch_requests=64,
+    srv = hcx.Server(hcx.default_opts(precision=_prec(args)), hbm_budget_bytes=4 << 30, max_batch_requests=64,
                      streams=4)
     tids = [srv.add_type(p) for p in probs]
     for ti, ty in enumerate(types):
@@ -753,7 +758,7 @@ def bench_type_sweep(args):
     from hcx import workloads
     B = 64 if args.instances == 1024 else args.instances
     types = workloads.quadratic_types(workloads.CONFIG5_TYPES, seed=5)
-    opts = hcx.default_opts(precision=hcx.PREC_MIXED)
+    opts = hcx.default_opts(precision=_prec(args))
     stream = torch.cuda.current_stream()
     rows = []
     for ty in types:

@@ -21,6 +21,11 @@ constexpr int kMaxN = 32;
 #define HCX_TERM_CHUNK 4
 #endif
 constexpr int kTermChunk = HCX_TERM_CHUNK;
+// terms of an LDS-table row per unrolled loop iteration: two in FP64 (the
+// loads in flight then fit a budget of 168 VGPRs, three waves per SIMD),
+// four in FP32
+template <class T>
+constexpr int kTermUnroll = sizeof(T) == 8 ? 2 : 4;
 
 // Lanes per path of the size-specialised instance for n: n, except n = 30,
 // which runs on aligned 32-lane groups (still two paths per wave) so that
@@ -118,7 +123,7 @@ __device__ __forceinline__ void eval_jac_row(const Seg& S, int gle, const cx<T>*
       const int cnt = S.hx_cnt[j];
       if constexpr (LT) {
         const uint2* tp = lt + (size_t)j * S.hxT * n + gle;
-#pragma unroll 4
+#pragma unroll kTermUnroll<T>
         for (int k = 0; k < cnt; ++k) {
           const uint2 tm = tp[(size_t)k * n];
           cx<T> v = cscale(sc[cterm_c(tm)], (T)cterm_s(tm));
@@ -162,7 +167,7 @@ __device__ __forceinline__ cx<T> eval_t_row(const Seg& S, int gle, const cx<T>* 
   cx<T> acc = cmk<T>(0, 0);
   if constexpr (LT) {
     const uint2* tp = lt + (size_t)n * S.hxT * n + gle;
-#pragma unroll 4
+#pragma unroll kTermUnroll<T>
     for (int k = 0; k < S.htT; ++k) {
       const uint2 tm = tp[(size_t)k * n];
       const int c = cterm_c(tm);
@@ -215,7 +220,7 @@ __device__ __forceinline__ void stage_coeffs(const Seg& S, const Group& g, bool 
   const double2* ht = S.cht + (size_t)inst * C1 * M;
   if (M == 1) {
     // c(t) = c0 + t c1, dc/dt = Ht[j][0]
-#pragma unroll 4
+#pragma unroll kTermUnroll<T>
     for (int j = g.gl; j < C1; j += g.G) {
       const int q = S.cslot[j];
       const double2 a = hx[2 * (size_t)j], b = hx[2 * (size_t)j + 1];
@@ -301,6 +306,35 @@ __device__ __forceinline__ void group_argmax(T& v, int& key, const Group& g) {
   }
 }
 
+// Sums over the group's rows of two per-row values (the corrector's and the
+// refinement's |d|^2 and |x|^2), returned to every lane of the group.  Aligned
+// power-of-two groups use lane exchanges; other group sizes publish the
+// pairs in the group's LDS slots `q` and every lane adds them in row order
+// (one LDS round trip).
+template <int NMAX, int GC>
+__device__ __forceinline__ void group_sum2(double& a, double& b, const Group& g, int n, bool row, int lane,
+                                           cx<double>* q) {
+  if constexpr (GC > 0) {
+    a = group_sum(a, g, lane);
+    b = group_sum(b, g, lane);
+  } else {
+    if (row) q[g.gl] = cmk<double>(a, b);
+    HCX_WAVE_SYNC();
+    double sa = 0.0, sb = 0.0;
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+      if (j < n) {
+        const cx<double> v = q[j];
+        sa += v.re;
+        sb += v.im;
+      }
+    }
+    HCX_WAVE_SYNC();
+    a = sa;
+    b = sb;
+  }
+}
+
 // ------------------------------------------------------------ group solve
 // Solves A x = b for the group's n x n system, row r held by lane r (A[j] =
 // A[r][j]).  Returns x_gl to lane gl.  Pivoting is the reference's partial
@@ -333,12 +367,38 @@ __device__ __forceinline__ cx<T> group_lu_solve(cx<T> (&A)[NMAX], cx<T> b, const
     if (k < n) {
       T v = (row && label >= k) ? cabs1(A[k]) : T(-1);
       if (v != v) v = T(INFINITY);
-      int key = (label << 6) | g.gl;
-      group_argmax<GC>(v, key, g);
-      const int bl = key & 63, blabel = key >> 6;
-      const bool zero = !(v > T(0));
+      int blabel;
+      bool piv, zero;
+      if constexpr (GC > 0) {
+        int key = (label << 6) | g.gl;
+        group_argmax<GC>(v, key, g);
+        blabel = key >> 6;
+        zero = !(v > T(0));
+        piv = g.gl == (key & 63);
+      } else {
+        // other group sizes: the candidates publish their magnitudes by
+        // position label and every lane scans them in current row order
+        // (one LDS round trip instead of a ring of lane exchanges)
+        T* mag = reinterpret_cast<T*>(sp);
+        if (row && label >= k) mag[label] = v;
+        HCX_WAVE_SYNC();
+        T best = mag[k];
+        blabel = k;
+#pragma unroll
+        for (int j = k + 1; j < NMAX; ++j) {
+          if (j < n) {
+            const T mj = mag[j];
+            if (mj > best) {
+              best = mj;
+              blabel = j;
+            }
+          }
+        }
+        HCX_WAVE_SYNC();
+        zero = !(best > T(0));
+        piv = row && label == blabel;
+      }
       singular = singular || zero;
-      const bool piv = g.gl == bl;
       if (g.in_grp && piv) {
 #pragma unroll
         for (int j = k; j < NMAX; ++j)
@@ -768,6 +828,352 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
   }
 }
 
+// Right-hand side row of the FP64 micro-step loop: sum_k s * coef * prod x
+// over the dHdt table (…cuh:89-120), with coef = c(t) from the group's LDS
+// slots (H) or, for the lanes of groups in an RK4 stage (dot), dc/dt from
+// `dtab`: LDS slots next to c(t), or (GDOT) the instance's table in global
+// memory indexed by coefficient number (cinv maps slots to coefficients).
+template <int DEG, bool LT, bool GDOT>
+__device__ __forceinline__ cx<double> eval_rhs_sel(const Seg& S, int gle, const cx<double>* sx,
+                                                   const cx<double>* sc, const cx<double>* dtab, bool dot,
+                                                   const uint2* lt) {
+  const int n = S.n;
+  const cx<double>* base = dot ? dtab : sc;
+  cx<double> acc = cmk<double>(0, 0);
+  if constexpr (LT) {
+    const uint2* tp = lt + (size_t)n * S.hxT * n + gle;
+#pragma unroll kTermUnroll<double>
+    for (int k = 0; k < S.htT; ++k) {
+      const uint2 tm = tp[(size_t)k * n];
+      int c = cterm_c(tm);
+      if (GDOT && dot) c = S.cinv[c];
+      cx<double> v = cscale(base[c], (double)cterm_s(tm));
+#pragma unroll
+      for (int q = 0; q < DEG; ++q) v = cmul(v, sx[cterm_var(tm, q)]);
+      acc = cadd(acc, v);
+    }
+  } else {
+    const Term* tp = S.ht + gle;
+    for (int k0 = 0; k0 < S.htT; k0 += kTermChunk) {
+      Term tm[kTermChunk];
+      int ci[kTermChunk];
+#pragma unroll
+      for (int q = 0; q < kTermChunk; ++q)
+        if (k0 + q < S.htT) tm[q] = tp[(size_t)(k0 + q) * n];
+#pragma unroll
+      for (int q = 0; q < kTermChunk; ++q)
+        if (k0 + q < S.htT) ci[q] = (GDOT && dot) ? S.cinv[tm[q].c] : tm[q].c;
+#pragma unroll
+      for (int q = 0; q < kTermChunk; ++q) {
+        if (k0 + q < S.htT) {
+          cx<double> v = cscale(base[ci[q]], (double)tm[q].s);
+#pragma unroll
+          for (int d = 0; d < DEG; ++d) v = cmul(v, sx[term_var(tm[q], d)]);
+          acc = cadd(acc, v);
+        }
+      }
+    }
+  }
+  return acc;
+}
+
+// Phases of a lane group in the FP64 micro-step loop.
+enum : int {
+  kPhIdle = 0,
+  kPhLoop,     // at the loop top of the step loop (…4pts.cu:116)
+  kPhStage0,   // RK4 stages 0..3 (…4pts.cu:139-183)
+  kPhStage1,
+  kPhStage2,
+  kPhStage3,
+  kPhNewton,   // corrector iteration at the new t (…4pts.cu:189-223)
+  kPhRefine,   // refinement iteration on H(x, 1)
+  kPhResid,    // residual H(x, 1) of the refined root
+  kPhOut,      // outputs pending
+};
+
+// FP64 lane-group path loop in uniform micro-steps.  Every iteration of the
+// wave performs ONE evaluation (Jacobian rows and a right-hand side) and ONE
+// group solve for every active group, whatever each group's path is doing:
+// an RK4 stage, a corrector iteration, a refinement iteration or the final
+// residual.  A group that leaves a phase (a corrector that converges after
+// one iteration, a path that ends) takes its next micro-step in the next
+// iteration, so no group waits for the slowest group of its wave to finish a
+// phase.  Each path's sequence of operations and results are those of the
+// step loop (…4pts.cu:115-254) and of the refinement: only the interleaving
+// of the groups of a wave changes.  c(t) is staged in LDS only when the
+// group's evaluation point moves (RK4 stages 0, 1 and 3; the corrector
+// reuses stage 3's c(t + dt)).
+template <int NMAX, int DEG, bool EXACT>
+__device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsigned char* smem, int lane) {
+  Seg S = LN.seg[si];
+  constexpr int GE = exact_lanes(NMAX);
+  if (EXACT) {
+    S.n = NMAX;
+    S.G = GE;
+  }
+  constexpr int GC = (EXACT && is_pow2(GE)) ? GE : 0;
+  const Opts o = LN.o;
+  const Group g = make_group(S.G, lane);
+  const int n = S.n;
+  const bool row = g.in_grp && g.gl < n;
+  const int gle = row ? g.gl : 0;
+  constexpr bool LT = EXACT && NMAX <= 16;
+  const uint2* lt = nullptr;
+  if constexpr (LT) {
+    uint2* d = reinterpret_cast<uint2*>(smem);
+    const int words = (S.hxT * n + S.htT) * n;
+    for (int i = lane; i < words; i += 64) d[i] = S.ctab[i];
+    HCX_WAVE_SYNC();
+    lt = d;
+  }
+  Lds<NMAX> L{smem + LN.tab_lds + (size_t)(g.in_grp ? g.grp : 0) * S.lds_group};
+  cx<double>* sx = L.template x<double>();
+  const cx<double>* sc = reinterpret_cast<const cx<double>*>(L.coef());
+  // dc/dt of the RK4 stages: constant (M = 1) -> read from the instance's
+  // table in global memory (half the group's LDS), unless the launch stages
+  // it in LDS (dot_lds: launches of few paths, latency bound); M > 1 -> LDS
+  const bool gdot = S.M == 1 && !LN.dot_lds;
+
+  long long path = -1;
+  int inst = 0, root = 0;
+  double t = 0.0, dt = 0.0, ts = 0.0, tst = -1.0, resid = 0.0;
+  int cnt = 0, nsteps = 0, asteps = 0, nnewt = 0, nref = 0, nit = 0;
+  bool ez = false, conv = false, sing_any = false;
+  int ph = kPhIdle;
+  uint32_t fl = 0;
+  bool exhausted = !g.in_grp;
+  long long static_path = (long long)((int)blockIdx.x - LN.first_block[si]) * g.gpw + g.grp;
+  cx<double> x = cmk<double>(0, 0), xl = x, acc = x, kp = x;
+
+  for (;;) {
+    // ------------------------------------------------ refill idle groups
+    const bool want = ph == kPhIdle && !exhausted;
+    if (__any(want)) {
+      long long p;
+      if (LN.persistent) {
+        unsigned int q = 0;
+        if (want && g.gl == 0) {
+          q = atomicAdd(S.counter, 1u);
+          if (S.plist) q = q < *S.plist_count ? (unsigned int)S.plist[q] : 0xffffffffu;
+        }
+        p = (long long)(unsigned int)lane_read((int)q, g.gbase);
+        if (p == 0xffffffffll) p = -1;
+      } else {
+        p = static_path;
+      }
+      if (want) {
+        if (p < 0 || p >= S.P) {
+          exhausted = true;
+        } else {
+          path = p;
+          inst = (int)(p / S.K);
+          root = (int)(p - (long long)inst * S.K);
+          nsteps = asteps = nnewt = nref = cnt = 0;
+          fl = S.plist ? HCX_FLAG_ESCALATED : 0u;
+          t = 0.0;
+          dt = o.dt0;
+          ez = false;
+          tst = -1.0;  // the LDS coefficients belong to another instance
+          resid = 0.0;
+          if (row) {
+            const double2 v = S.start[(size_t)root * S.start_stride + g.gl];
+            x = cmk<double>(v.x, v.y);
+          } else {
+            x = cmk<double>(0, 0);
+          }
+          xl = x;
+          ph = kPhLoop;
+          if (!LN.persistent) exhausted = true;
+        }
+      }
+    }
+    if (!__any(ph != kPhIdle)) break;
+
+    // ------------------------------------------------ loop top (:116-131)
+    bool fin = false;
+    if (ph == kPhLoop) {
+      if (!(t < 1.0 && 1.0 - t > o.t_tol)) {
+        fl |= HCX_FLAG_TRACKED;
+        fin = true;
+      } else if (asteps > o.max_steps) {
+        fl |= HCX_FLAG_BUDGET;
+        fin = true;
+      } else {
+        if (!ez && fabs(1.0 - t) <= o.ez_enter) ez = true;
+        if (ez) {
+          if (dt > fabs(1.0 - t)) dt = fabs(1.0 - t);
+        } else if (dt > fabs(o.ez_land - t)) {
+          dt = fabs(o.ez_land - t);
+        }
+        ts = t;
+        acc = cmk<double>(0, 0);
+        kp = acc;
+        ph = kPhStage0;
+      }
+    }
+    if (__any(fin)) {
+      // the reference's real test on the tracked root, before refinement
+      // (Evaluations.cpp:96-104: converged and every |Im x_i| <= 1e-4)
+      double imt = (fin && row) ? fabs(x.im) : 0.0;
+      if (imt != imt) imt = INFINITY;
+      imt = group_max(imt, g);
+      if (fin) {
+        if ((fl & HCX_FLAG_TRACKED) && imt <= o.real_ref_tol) fl |= HCX_FLAG_REAL_REF;
+        if ((fl & HCX_FLAG_TRACKED) && o.refine_iters > 0) {
+          ph = kPhRefine;
+          nit = 0;
+          conv = false;
+          sing_any = false;
+        } else {
+          ph = kPhOut;
+        }
+      }
+    }
+
+    // ------------------------------------------------ one micro-step
+    const int pe = ph;
+    const bool stg = pe >= kPhStage0 && pe <= kPhStage3;
+    const bool act = stg || pe == kPhNewton || pe == kPhRefine || pe == kPhResid;
+    if (__any(act)) {
+      const int k = pe - kPhStage0;
+      const double a = k == 0 ? 0.0 : (k == 3 ? dt : 0.5 * dt);
+      const double te = stg ? ts + a : (pe == kPhNewton ? t : 1.0);
+      const cx<double> xe = stg ? cmk<double>(xl.re + a * kp.re, xl.im + a * kp.im) : x;
+      const bool restage = act && te != tst;
+      if (__any(restage)) stage_coeffs<double>(S, g, restage, inst, te, L.coef(), !gdot);
+      if (restage) tst = te;
+      publish_x<double>(sx, xe, g, n, act, row);
+      HCX_WAVE_SYNC();
+      cx<double> A[NMAX];
+      eval_jac_row<NMAX, DEG, double, LT>(S, gle, sx, sc, A, lt);
+      cx<double> r;
+      if (gdot) {
+        const cx<double>* dg = reinterpret_cast<const cx<double>*>(S.cht + (size_t)inst * (S.C + 1));
+        r = eval_rhs_sel<DEG, LT, true>(S, gle, sx, sc, dg, stg, lt);
+      } else {
+        r = eval_rhs_sel<DEG, LT, false>(S, gle, sx, sc, sc + (S.C + 1), stg, lt);
+      }
+      bool sing;
+      const cx<double> d =
+          group_lu_solve<NMAX, GC, double>(A, stg ? cmk<double>(-r.re, -r.im) : r, g, n, row, L.template piv<double>(), sing);
+
+      // RK4 stage: k_s = solution, accumulate k1 + 2 k2 + 2 k3 + k4
+      if (stg) {
+        kp = d;
+        const double w = (k == 1 || k == 2) ? 2.0 : 1.0;
+        acc = cmk<double>(acc.re + w * kp.re, acc.im + w * kp.im);
+        if (k == 3) {
+          const double f = dt * (1.0 / 6.0);
+          x = cmk<double>(xl.re + f * acc.re, xl.im + f * acc.im);
+          t = ts + dt;
+          ph = kPhNewton;
+          nit = 0;
+        } else {
+          ph = pe + 1;
+        }
+      }
+      // corrector and refinement iterations: x -= d and the group norms
+      const bool nr = pe == kPhNewton || pe == kPhRefine;
+      if (nr && row) x = csub(x, d);
+      double s1 = (nr && row) ? cnorm2(d) : 0.0, s2 = (nr && row) ? cnorm2(x) : 0.0;
+      if (__any(nr)) group_sum2<NMAX, GC>(s1, s2, g, n, row, lane, L.template piv<double>());
+      if (pe == kPhNewton) {
+        ++nnewt;
+        ++nit;
+        const bool ok = s1 < o.corr_tol * s2, inf = s2 > o.inf_tol;
+        if (ok || inf || nit >= o.max_corr) {
+          // step control (:225-249)
+          ++nsteps;
+          ++asteps;
+          if (inf) {
+            fl |= HCX_FLAG_INFINITY;
+            ph = kPhOut;
+          } else if (!ok) {
+            dt *= 0.5;
+            x = xl;
+            t = ts;
+            cnt = 0;
+            if (dt < o.dt_min) {
+              fl |= HCX_FLAG_MINSTEP;
+              ph = kPhOut;
+            } else {
+              ph = kPhLoop;
+            }
+          } else {
+            ++cnt;
+            xl = x;
+            if (cnt >= o.incr) {
+              cnt = 0;
+              dt *= 2.0;
+            }
+            ph = kPhLoop;
+          }
+        }
+      } else if (pe == kPhRefine) {
+        ++nref;
+        ++nit;
+        sing_any = sing;
+        bool stop = false;
+        if (!(s2 < o.inf_tol)) {
+          stop = true;
+        } else if (sqrt(s1) <= o.refine_tol * fmax(1.0, sqrt(s2))) {
+          conv = true;
+          stop = true;
+        }
+        if (stop || nit >= o.refine_iters) ph = kPhResid;
+      }
+      // residual max_i |H_i(x, 1)| of the refined root (r = H at x)
+      const bool rz = pe == kPhResid;
+      if (__any(rz)) {
+        double rr = (rz && row) ? sqrt(cnorm2(r)) : 0.0;
+        if (rr != rr) rr = INFINITY;
+        rr = group_max(rr, g);
+        double im = (rz && row) ? fabs(x.im) : 0.0;
+        if (im != im) im = INFINITY;
+        im = group_max(im, g);
+        if (rz) {
+          resid = rr;
+          if (conv) fl |= HCX_FLAG_REFINED;
+          if (conv && im <= o.real_tol) fl |= HCX_FLAG_REAL;
+          if (sing_any) fl |= HCX_FLAG_SINGULAR;
+          ph = kPhOut;
+        }
+      }
+      HCX_WAVE_SYNC();
+    }
+
+    // ------------------------------------------------ outputs
+    const bool out = ph == kPhOut;
+    if (__any(out) && out) {
+      const SegOut* O = S.out;
+      double2* roots = O->roots;
+      const int rstride = O->roots_stride;
+      if (row) roots[(size_t)path * rstride + g.gl] = make_double2(x.re, x.im);
+      if (g.gl == 0) {
+        if (rstride > n) roots[(size_t)path * rstride + n] = make_double2(1.0, 0.0);
+        if (O->flags) O->flags[path] = fl;
+        if (O->steps) O->steps[path] = nsteps;
+        if (O->newton) O->newton[path] = nnewt;
+        if (O->refine) O->refine[path] = nref;
+        if (O->tfin) O->tfin[path] = t;
+        if (O->dtfin) O->dtfin[path] = dt;
+        if (O->resid) O->resid[path] = resid;
+        if (O->conv8) O->conv8[path] = (fl & HCX_FLAG_TRACKED) ? 1 : 0;
+        if (O->inf8) O->inf8[path] = (fl & HCX_FLAG_INFINITY) ? 1 : 0;
+        if (O->debug) O->debug[path] = (fl & HCX_FLAG_TRACKED) ? make_double2(1.0, 0.0) : make_double2(t, dt);
+        if (O->inst_left) {
+          const int left = atomicSub(&O->inst_left[inst], 1);
+          if (left == 1 && O->inst_done) O->inst_done[inst] = wall_clock64();
+        } else if (O->inst_done && S.plist) {
+          atomicMax(&O->inst_done[inst], (unsigned long long)wall_clock64());
+        }
+      }
+      ph = kPhIdle;
+      path = -1;
+    }
+  }
+}
+
 template <int NMAX, int DEG, int MODE, bool EXACT>
 __global__ __launch_bounds__(64) void track_kernel(const Launch LN) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -775,7 +1181,11 @@ __global__ __launch_bounds__(64) void track_kernel(const Launch LN) {
   int s = 0;
   while (s + 1 < LN.nseg && (int)blockIdx.x >= LN.first_block[s + 1]) ++s;
   for (int tried = 0; tried < LN.nseg; ++tried) {
-    run_segment<NMAX, DEG, MODE, EXACT>(LN, s, smem, lane);
+    if constexpr (MODE == 0) {
+      run_segment_fp64<NMAX, DEG, EXACT>(LN, s, smem, lane);
+    } else {
+      run_segment<NMAX, DEG, MODE, EXACT>(LN, s, smem, lane);
+    }
     if (!LN.persistent) break;
     s = s + 1 == LN.nseg ? 0 : s + 1;
   }
