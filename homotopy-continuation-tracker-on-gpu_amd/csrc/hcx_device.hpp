@@ -13,8 +13,11 @@
 namespace hcx {
 namespace dev {
 
+// Complex value; aligned to its size, so that one 16-byte (FP64) or 8-byte
+// (FP32) LDS or global access moves it (no split into two half accesses,
+// which would touch every other LDS bank twice in a lane-contiguous gather).
 template <class T>
-struct cx {
+struct alignas(2 * sizeof(T)) cx {
   T re, im;
 };
 

@@ -104,18 +104,48 @@ struct Launch {
   int persistent;
   int tab_lds;                 // bytes of dynamic LDS before the group regions (compact tables)
   int dot_lds;                 // FP64 mode, M = 1: stage dc/dt in LDS (else read it from global)
+  int wpb;                     // waves per block (> 1 only for one-segment launches: the
+                               // waves of a block share one copy of the compact tables)
   Opts o;
 };
 
+// The options and segment descriptors in the kernel-argument (constant)
+// address space: loads through these types are scalar loads.
+typedef const __attribute__((address_space(4))) Opts KOpts;
+typedef const __attribute__((address_space(4))) Seg KSeg;
+
 namespace dev {
+
+// The launch's options where the kernel argument holds them (the
+// descriptor is the kernels' only argument: offset 0 of the segment).
+__device__ __forceinline__ const KOpts* launch_opts() {
+  return (const KOpts*)((const __attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr() +
+                        offsetof(Launch, o));
+}
+
+// Segment si's descriptor where the kernel argument holds it.
+__device__ __forceinline__ const KSeg* launch_seg(int si) {
+  return (const KSeg*)((const __attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr() +
+                       offsetof(Launch, seg) + (size_t)si * sizeof(Seg));
+}
+
+// A kernel-argument view for one iteration of a path loop: the pointer
+// passes through an empty asm keyed on the iteration count, so the fields
+// are re-read (scalar loads) where an iteration uses them instead of
+// staying live in scalar registers across the loop, where they would be
+// spilled to vector-register lanes and reloaded with VALU instructions.
+template <class T>
+__device__ __forceinline__ T* iteration_view(T* p, int iter) {
+  asm("" : "+s"(p) : "s"(iter));
+  return p;
+}
 
 // ------------------------------------------------------------ evaluation
 // Row gl of dH/dx (dev-eval-indxing-generalized_3views_4pts.cuh:70-86).
 // LT: the tables are the compact copy in LDS at lt (hx part first).
-template <int NMAX, int DEG, class T, bool LT>
-__device__ __forceinline__ void eval_jac_row(const Seg& S, int gle, const cx<T>* sx, const cx<T>* sc,
+template <int NMAX, int DEG, class T, bool LT, class SegT>
+__device__ __forceinline__ void eval_jac_row(const SegT& S, int n, int gle, const cx<T>* sx, const cx<T>* sc,
                                              cx<T> (&A)[NMAX], const uint2* lt) {
-  const int n = S.n;
 #pragma unroll
   for (int j = 0; j < NMAX; ++j) {
     cx<T> acc = cmk<T>(0, 0);
@@ -160,10 +190,9 @@ __device__ __forceinline__ void eval_jac_row(const Seg& S, int gle, const cx<T>*
 // dH/dt when coef = dc/dt (…cuh:89-120).
 // coef is an LDS table indexed by slot, or (remap != nullptr) a global table
 // indexed by coefficient.
-template <int DEG, class T, bool LT>
-__device__ __forceinline__ cx<T> eval_t_row(const Seg& S, int gle, const cx<T>* sx, const cx<T>* coef,
+template <int DEG, class T, bool LT, class SegT>
+__device__ __forceinline__ cx<T> eval_t_row(const SegT& S, int n, int gle, const cx<T>* sx, const cx<T>* coef,
                                             const uint2* lt, const int* remap = nullptr) {
-  const int n = S.n;
   cx<T> acc = cmk<T>(0, 0);
   if constexpr (LT) {
     const uint2* tp = lt + (size_t)n * S.hxT * n + gle;
@@ -209,8 +238,8 @@ __device__ __forceinline__ cx<T> eval_t_row(const Seg& S, int gle, const cx<T>* 
 // rows of a group read them (term by term), so the lanes of a group that read
 // term k of their rows together hit consecutive LDS words instead of words one
 // equation apart (a bank conflict per lane pair when equations have many terms).
-template <class T>
-__device__ __forceinline__ void stage_coeffs(const Seg& S, const Group& g, bool act, int inst, double t,
+template <class T, class SegT>
+__device__ __forceinline__ void stage_coeffs(const SegT& S, const Group& g, bool act, int inst, double t,
                                              unsigned char* scb, bool with_dot) {
   if (!act) return;
   cx<T>* sc = reinterpret_cast<cx<T>*>(scb);
@@ -429,6 +458,18 @@ __device__ __forceinline__ cx<T> group_lu_solve(cx<T> (&A)[NMAX], cx<T> b, const
   return sol;
 }
 
+// Copies a segment's compact term tables to the start of the dynamic LDS.
+// Every wave of the block takes part (one copy per block) and the block
+// barrier orders every wave's table reads after the copy; blocks of several
+// waves run one segment only, so each wave reaches the barrier once.
+__device__ __forceinline__ const uint2* stage_tables(const Seg& S, int n, unsigned char* smem) {
+  uint2* d = reinterpret_cast<uint2*>(smem);
+  const int words = (S.hxT * n + S.htT) * n;
+  for (int i = (int)threadIdx.x; i < words; i += (int)blockDim.x) d[i] = S.ctab[i];
+  __syncthreads();
+  return d;
+}
+
 // Per-group view of the LDS region.  x (read by the evaluation) and the
 // pivot-row buffer (written by the LU that follows it) are never live at the
 // same time, so they share the first NMAX + 1 slots; group_lu_solve starts
@@ -449,8 +490,8 @@ struct Lds {
 // coefficients c(t) must already be in the group's LDS.  `run` enters as the
 // set of groups to correct; ok / inf return the reference's r_isSuccessful /
 // r_isInfFail.
-template <int NMAX, int GC, int DEG, class T, bool LT>
-__device__ __forceinline__ void newton_pass(const Seg& S, const Opts& o, const Group& g, const Lds<NMAX>& L,
+template <int NMAX, int GC, int DEG, class T, bool LT, class OT>
+__device__ __forceinline__ void newton_pass(const Seg& S, const OT& o, const Group& g, const Lds<NMAX>& L,
                                             const uint2* lt, bool run, bool row, cx<double>& x, int& nnewt, bool& ok,
                                             bool& inf) {
   ok = false;
@@ -464,15 +505,15 @@ __device__ __forceinline__ void newton_pass(const Seg& S, const Opts& o, const G
     publish_x<T>(sx, x, g, n, run, row);
     HCX_WAVE_SYNC();
     cx<T> A[NMAX];
-    eval_jac_row<NMAX, DEG, T, LT>(S, gle, sx, sc, A, lt);
-    const cx<T> h = eval_t_row<DEG, T, LT>(S, gle, sx, sc, lt);
+    eval_jac_row<NMAX, DEG, T, LT>(S, n, gle, sx, sc, A, lt);
+    const cx<T> h = eval_t_row<DEG, T, LT>(S, n, gle, sx, sc, lt);
     bool sing;
     const cx<T> d = group_lu_solve<NMAX, GC, T>(A, h, g, n, row, L.template piv<T>(), sing);
     if (run && row) x = csub(x, ccast<double>(d));
     double rs = row ? (double)cnorm2(d) : 0.0;
     double rc = row ? cnorm2(x) : 0.0;
-    rs = group_sum(rs, g, threadIdx.x);
-    rc = group_sum(rc, g, threadIdx.x);
+    rs = group_sum(rs, g, g.gbase + g.gl);
+    rc = group_sum(rc, g, g.gbase + g.gl);
     if (run) {
       ++nnewt;
       ok = rs < o.corr_tol * rc;
@@ -509,8 +550,8 @@ __device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, cons
     publish_x<T>(sx, xs, g, n, act, row);
     HCX_WAVE_SYNC();
     cx<T> A[NMAX];
-    eval_jac_row<NMAX, DEG, T, LT>(S, gle, sx, sc, A, lt);
-    const cx<T> bt = eval_t_row<DEG, T, LT>(S, gle, sx, sd, lt, sd_ext ? S.cinv : nullptr);
+    eval_jac_row<NMAX, DEG, T, LT>(S, n, gle, sx, sc, A, lt);
+    const cx<T> bt = eval_t_row<DEG, T, LT>(S, n, gle, sx, sd, lt, sd_ext ? S.cinv : nullptr);
     bool sing;
     const cx<T> k = group_lu_solve<NMAX, GC, T>(A, cmk<T>(-bt.re, -bt.im), g, n, row, L.template piv<T>(), sing);
     kprev = ccast<double>(k);
@@ -556,16 +597,11 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
   // dependency chain, and LDS returns them several times sooner.
   constexpr bool LT = EXACT && NMAX <= 16;
   const uint2* lt = nullptr;
-  if constexpr (LT) {
-    uint2* d = reinterpret_cast<uint2*>(smem);
-    const int words = (S.hxT * n + S.htT) * n;
-    for (int i = lane; i < words; i += 64) d[i] = S.ctab[i];
-    HCX_WAVE_SYNC();
-    lt = d;
-  }
+  if constexpr (LT) lt = stage_tables(S, n, smem);
   // lanes outside the full groups (64 % G of them) alias region 0: they
   // track nothing and store nothing to LDS
-  Lds<NMAX> L{smem + LN.tab_lds + (size_t)(g.in_grp ? g.grp : 0) * S.lds_group};
+  const int wv = (int)threadIdx.x >> 6;  // wave of the block
+  Lds<NMAX> L{smem + LN.tab_lds + ((size_t)wv * g.gpw + (g.in_grp ? g.grp : 0)) * S.lds_group};
 
   // per-group path state, replicated over the group's lanes
   long long path = -1;
@@ -576,7 +612,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
   int status = 0;  // 0 idle, 1 tracking, 2 finishing
   uint32_t fl = 0;
   bool exhausted = !g.in_grp;
-  long long static_path = (long long)((int)blockIdx.x - LN.first_block[si]) * g.gpw + g.grp;
+  long long static_path = ((long long)((int)blockIdx.x - LN.first_block[si]) * LN.wpb + wv) * g.gpw + g.grp;
   // lane-distributed state: component gl of x and of the last accepted x
   cx<double> x = cmk<double>(0, 0), xl = x;
 
@@ -752,8 +788,8 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
           publish_x<double>(sx, x, g, n, run, row);
           HCX_WAVE_SYNC();
           cx<double> A[NMAX];
-          eval_jac_row<NMAX, DEG, double, LT>(S, gle, sx, sc, A, lt);
-          const cx<double> hv = eval_t_row<DEG, double, LT>(S, gle, sx, sc, lt);
+          eval_jac_row<NMAX, DEG, double, LT>(S, n, gle, sx, sc, A, lt);
+          const cx<double> hv = eval_t_row<DEG, double, LT>(S, n, gle, sx, sc, lt);
           bool sing;
           const cx<double> d = group_lu_solve<NMAX, GC, double>(A, hv, g, n, row, L.template piv<double>(), sing);
           if (run && row) x = csub(x, d);
@@ -774,7 +810,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
         // residual max_i |H_i(x, 1)|
         publish_x<double>(sx, x, g, n, ref, row);
         HCX_WAVE_SYNC();
-        const cx<double> hv = eval_t_row<DEG, double, LT>(S, gle, sx, sc, lt);
+        const cx<double> hv = eval_t_row<DEG, double, LT>(S, n, gle, sx, sc, lt);
         double r = row ? sqrt(cnorm2(hv)) : 0.0;
         if (r != r) r = INFINITY;
         resid = group_max(r, g);
@@ -833,11 +869,10 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
 // slots (H) or, for the lanes of groups in an RK4 stage (dot), dc/dt from
 // `dtab`: LDS slots next to c(t), or (GDOT) the instance's table in global
 // memory indexed by coefficient number (cinv maps slots to coefficients).
-template <int DEG, bool LT, bool GDOT>
-__device__ __forceinline__ cx<double> eval_rhs_sel(const Seg& S, int gle, const cx<double>* sx,
+template <int DEG, bool LT, bool GDOT, class SegT>
+__device__ __forceinline__ cx<double> eval_rhs_sel(const SegT& S, int n, int gle, const cx<double>* sx,
                                                    const cx<double>* sc, const cx<double>* dtab, bool dot,
                                                    const uint2* lt) {
-  const int n = S.n;
   const cx<double>* base = dot ? dtab : sc;
   cx<double> acc = cmk<double>(0, 0);
   if constexpr (LT) {
@@ -912,21 +947,18 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
     S.G = GE;
   }
   constexpr int GC = (EXACT && is_pow2(GE)) ? GE : 0;
-  const Opts o = LN.o;
+  const KOpts* const opts_arg = launch_opts();
+  const KSeg* const seg_arg = launch_seg(si);
+  int iter = 0;
   const Group g = make_group(S.G, lane);
   const int n = S.n;
   const bool row = g.in_grp && g.gl < n;
   const int gle = row ? g.gl : 0;
   constexpr bool LT = EXACT && NMAX <= 16;
   const uint2* lt = nullptr;
-  if constexpr (LT) {
-    uint2* d = reinterpret_cast<uint2*>(smem);
-    const int words = (S.hxT * n + S.htT) * n;
-    for (int i = lane; i < words; i += 64) d[i] = S.ctab[i];
-    HCX_WAVE_SYNC();
-    lt = d;
-  }
-  Lds<NMAX> L{smem + LN.tab_lds + (size_t)(g.in_grp ? g.grp : 0) * S.lds_group};
+  if constexpr (LT) lt = stage_tables(S, n, smem);
+  const int wv = (int)threadIdx.x >> 6;  // wave of the block
+  Lds<NMAX> L{smem + LN.tab_lds + ((size_t)wv * g.gpw + (g.in_grp ? g.grp : 0)) * S.lds_group};
   cx<double>* sx = L.template x<double>();
   const cx<double>* sc = reinterpret_cast<const cx<double>*>(L.coef());
   // dc/dt of the RK4 stages: constant (M = 1) -> read from the instance's
@@ -942,10 +974,13 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
   int ph = kPhIdle;
   uint32_t fl = 0;
   bool exhausted = !g.in_grp;
-  long long static_path = (long long)((int)blockIdx.x - LN.first_block[si]) * g.gpw + g.grp;
+  long long static_path = ((long long)((int)blockIdx.x - LN.first_block[si]) * LN.wpb + wv) * g.gpw + g.grp;
   cx<double> x = cmk<double>(0, 0), xl = x, acc = x, kp = x;
 
   for (;;) {
+    const KOpts& o = *iteration_view(opts_arg, iter);
+    const KSeg& Sk = *iteration_view(seg_arg, iter);
+    ++iter;
     // ------------------------------------------------ refill idle groups
     const bool want = ph == kPhIdle && !exhausted;
     if (__any(want)) {
@@ -953,8 +988,8 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
       if (LN.persistent) {
         unsigned int q = 0;
         if (want && g.gl == 0) {
-          q = atomicAdd(S.counter, 1u);
-          if (S.plist) q = q < *S.plist_count ? (unsigned int)S.plist[q] : 0xffffffffu;
+          q = atomicAdd(Sk.counter, 1u);
+          if (Sk.plist) q = q < *Sk.plist_count ? (unsigned int)Sk.plist[q] : 0xffffffffu;
         }
         p = (long long)(unsigned int)lane_read((int)q, g.gbase);
         if (p == 0xffffffffll) p = -1;
@@ -962,21 +997,21 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
         p = static_path;
       }
       if (want) {
-        if (p < 0 || p >= S.P) {
+        if (p < 0 || p >= Sk.P) {
           exhausted = true;
         } else {
           path = p;
-          inst = (int)(p / S.K);
-          root = (int)(p - (long long)inst * S.K);
+          inst = (int)(p / Sk.K);
+          root = (int)(p - (long long)inst * Sk.K);
           nsteps = asteps = nnewt = nref = cnt = 0;
-          fl = S.plist ? HCX_FLAG_ESCALATED : 0u;
+          fl = Sk.plist ? HCX_FLAG_ESCALATED : 0u;
           t = 0.0;
           dt = o.dt0;
           ez = false;
           tst = -1.0;  // the LDS coefficients belong to another instance
           resid = 0.0;
           if (row) {
-            const double2 v = S.start[(size_t)root * S.start_stride + g.gl];
+            const double2 v = Sk.start[(size_t)root * Sk.start_stride + g.gl];
             x = cmk<double>(v.x, v.y);
           } else {
             x = cmk<double>(0, 0);
@@ -1040,18 +1075,18 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
       const double te = stg ? ts + a : (pe == kPhNewton ? t : 1.0);
       const cx<double> xe = stg ? cmk<double>(xl.re + a * kp.re, xl.im + a * kp.im) : x;
       const bool restage = act && te != tst;
-      if (__any(restage)) stage_coeffs<double>(S, g, restage, inst, te, L.coef(), !gdot);
+      if (__any(restage)) stage_coeffs<double>(Sk, g, restage, inst, te, L.coef(), !gdot);
       if (restage) tst = te;
       publish_x<double>(sx, xe, g, n, act, row);
       HCX_WAVE_SYNC();
       cx<double> A[NMAX];
-      eval_jac_row<NMAX, DEG, double, LT>(S, gle, sx, sc, A, lt);
+      eval_jac_row<NMAX, DEG, double, LT>(Sk, n, gle, sx, sc, A, lt);
       cx<double> r;
       if (gdot) {
-        const cx<double>* dg = reinterpret_cast<const cx<double>*>(S.cht + (size_t)inst * (S.C + 1));
-        r = eval_rhs_sel<DEG, LT, true>(S, gle, sx, sc, dg, stg, lt);
+        const cx<double>* dg = reinterpret_cast<const cx<double>*>(Sk.cht + (size_t)inst * (Sk.C + 1));
+        r = eval_rhs_sel<DEG, LT, true>(Sk, n, gle, sx, sc, dg, stg, lt);
       } else {
-        r = eval_rhs_sel<DEG, LT, false>(S, gle, sx, sc, sc + (S.C + 1), stg, lt);
+        r = eval_rhs_sel<DEG, LT, false>(Sk, n, gle, sx, sc, sc + (Sk.C + 1), stg, lt);
       }
       bool sing;
       const cx<double> d =
@@ -1145,7 +1180,7 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
     // ------------------------------------------------ outputs
     const bool out = ph == kPhOut;
     if (__any(out) && out) {
-      const SegOut* O = S.out;
+      const SegOut* O = Sk.out;
       double2* roots = O->roots;
       const int rstride = O->roots_stride;
       if (row) roots[(size_t)path * rstride + g.gl] = make_double2(x.re, x.im);
@@ -1164,7 +1199,7 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
         if (O->inst_left) {
           const int left = atomicSub(&O->inst_left[inst], 1);
           if (left == 1 && O->inst_done) O->inst_done[inst] = wall_clock64();
-        } else if (O->inst_done && S.plist) {
+        } else if (O->inst_done && Sk.plist) {
           atomicMax(&O->inst_done[inst], (unsigned long long)wall_clock64());
         }
       }
@@ -1174,10 +1209,14 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
   }
 }
 
+// Register budget: the size-specialised instances up to n = 6 are compiled
+// for three waves per SIMD (at most 168 VGPRs), which their live state fits
+// without spills.
 template <int NMAX, int DEG, int MODE, bool EXACT>
-__global__ __launch_bounds__(64) void track_kernel(const Launch LN) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((EXACT && NMAX <= 6) ? 3 : 1)))
+void track_kernel(const Launch LN) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int lane = threadIdx.x;
+  const int lane = (int)threadIdx.x & 63;
   int s = 0;
   while (s + 1 < LN.nseg && (int)blockIdx.x >= LN.first_block[s + 1]) ++s;
   for (int tried = 0; tried < LN.nseg; ++tried) {

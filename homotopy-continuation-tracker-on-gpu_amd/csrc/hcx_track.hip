@@ -548,13 +548,14 @@ void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, 
 }
 
 hcx_status finish_plan(Launch& LN, Plan& pl) {
+  if (LN.wpb < 1) LN.wpb = 1;
   size_t lds = 0;
   for (int s = 0; s < LN.nseg; ++s) {
     const Seg& S = LN.seg[s];
-    // one region per full group; the lanes of a partial group share
-    // region 0 and never write LDS
+    // one region per full group of every wave of the block; the lanes of a
+    // partial group share their wave's region 0 and never write LDS
     const int groups = 64 / S.G;
-    lds = std::max(lds, (size_t)groups * (size_t)S.lds_group);
+    lds = std::max(lds, (size_t)LN.wpb * (size_t)groups * (size_t)S.lds_group);
   }
   LN.tab_lds = pl.tab_lds;
   lds += (size_t)pl.tab_lds;
@@ -564,6 +565,34 @@ hcx_status finish_plan(Launch& LN, Plan& pl) {
                                 (int)lds));
   }
   return HCX_OK;
+}
+
+// Waves per block of a persistent one-segment launch whose instance holds
+// the compact tables in LDS: the waves of a block share one copy of the
+// tables, so blocks of several waves can leave room for more waves per CU.
+// The candidate (1, 2 or 4 waves) with the most resident waves per CU wins,
+// the smaller block on ties.
+hcx_status choose_waves_per_block(Launch& LN, Plan& pl, int& best_wpb, int& best_nb) {
+  best_wpb = 1;
+  best_nb = 0;
+  int best_waves = -1;
+  for (int wpb : {1, 2, 4}) {
+    LN.wpb = wpb;
+    Plan q = pl;
+    hcx_status st = finish_plan(LN, q);
+    if (st != HCX_OK) return st;
+    if (q.lds > (size_t)160 * 1024) continue;
+    int nb = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(pl.fn), 64 * wpb,
+                                                         q.lds));
+    if (nb * wpb > best_waves) {
+      best_waves = nb * wpb;
+      best_wpb = wpb;
+      best_nb = nb;
+    }
+  }
+  LN.wpb = best_wpb;
+  return finish_plan(LN, pl);
 }
 
 // Default lanes per path: those of the size-specialised instance (n, or 32
@@ -1119,22 +1148,34 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
     seg_groups[s] = (S.P + gpw - 1) / gpw;
     total_groups_needed += seg_groups[s];
   }
-  st = finish_plan(LN, pl);
+  LN.wpb = 1;
+  int occ_blocks = -1;  // blocks per CU at full occupancy, when already known
+  if (persistent && nseg == 1 && pl.tab_lds > 0) {
+    int wpb = 1;
+    st = choose_waves_per_block(LN, pl, wpb, occ_blocks);
+  } else {
+    st = finish_plan(LN, pl);
+  }
   if (st != HCX_OK) return st;
+  const int wpb = LN.wpb;
 
   // grid: persistent -> occupancy-limited, split over segments by work
   long long blocks;
   if (persistent) {
-    int per_cu = o->waves_per_cu;
+    int per_cu = o->waves_per_cu;  // waves per CU
     if (per_cu == 0 && all_tuned && nseg == 1) per_cu = probs[0]->tuned_wpc;
     if (per_cu <= 0) {
       // occupancy, less -waves_per_cu waves when that is negative (room for
       // the work of other solves in flight on other streams)
-      int nb = 0;
-      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(pl.fn), 64, pl.lds));
-      per_cu = std::max(nb + std::min(per_cu, 0), 1);
+      int nb = occ_blocks;
+      if (nb < 0)
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(pl.fn), 64 * wpb,
+                                                             pl.lds));
+      per_cu = std::max(nb * wpb + std::min(per_cu, 0), 1);
     }
-    blocks = std::min<long long>((long long)per_cu * g_cu, std::max<long long>(total_groups_needed, 1));
+    // blocks of wpb waves: waves per CU rounded up to whole blocks
+    const long long waves_needed = std::max<long long>(total_groups_needed, 1);
+    blocks = std::min<long long>((long long)((per_cu + wpb - 1) / wpb) * g_cu, (waves_needed + wpb - 1) / wpb);
     // proportional split by estimated work (paths * n^2 as a cost proxy)
     std::vector<double> w(nseg);
     double wsum = 0;
@@ -1166,7 +1207,7 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
   dim3 ig((max_inst + 255) / 256, nseg);
   hipLaunchKernelGGL(init_kernel, ig, dim3(256), 0, stream, LN, OP, batch_start);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(pl.fn, dim3((unsigned)blocks), dim3(64), pl.lds, stream, LN);
+  hipLaunchKernelGGL(pl.fn, dim3((unsigned)blocks), dim3(64 * wpb), pl.lds, stream, LN);
   HIP_TRY(hipGetLastError());
 
   // Mixed precision: re-track in FP64 from t = 0 every path that ended
@@ -1176,6 +1217,7 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
   if (o->precision == HCX_PREC_MIXED) {
     Launch L2 = LN;
     L2.persistent = 1;
+    L2.wpb = 1;
     OutPack OP2 = OP;
     for (int s = 0; s < nseg; ++s) {
       hcx_problem* p = probs[s];
