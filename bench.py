@@ -1,6 +1,6 @@
 """Benchmark: converged paths/s + p99 solve latency of the MI355X tracker,
-mixed-precision batched (one persistent launch per batch) versus the plain
-FP64 per-problem launch, on the same GPU, same systems, parameters and seeds.
+batched (one persistent launch per batch) versus the plain FP64 per-problem
+launch, on the same GPU, same systems, parameters and seeds.
 
 Workload (default): n=6 parameter homotopy, 12 monomials per equation, k=24
 parameters; 64 start roots from a seed total-degree solve at p0; 1024 target
@@ -8,7 +8,8 @@ instances -> 65,536 paths per step.  A step tracks every path of the batch and
 collects the counts (refined / real / unique).
 
 Prints ONE JSON line (rank 0).  `value` = converged (FP64-refined) paths per
-second of the mixed batched form with all inputs resident in HBM.
+second of the batched form (FP64 micro-step tracker by default; the mixed
+FP32/FP64 tracker is measured beside it) with all inputs resident in HBM.
 """
 import argparse
 import json
@@ -158,15 +159,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed",
-                    help="config 2: precision of the batched solve (the headline is mixed)")
-    ap.add_argument("--inflight", type=int, default=3,
+    ap.add_argument("--precision", choices=("mixed", "fp64"), default="fp64",
+                    help="precision of the batched solve of the headline (the other one is measured "
+                         "beside it), of the per-type sweep and of the serving leg")
+    ap.add_argument("--inflight", type=int, default=4,
                     help="configs 1, 2: solves in flight on separate streams (the duplicate re-track "
                          "and tail of one solve overlap the next ones)")
-    ap.add_argument("--waves-per-cu", type=int, default=-1,
-                    help="config 2: persistent grid density (waves per CU; < 0: occupancy less that many); "
-                         "one below the kernel's occupancy leaves slots for the previous solve's re-track "
-                         "and tail")
+    ap.add_argument("--waves-per-cu", type=int, default=0,
+                    help="config 2: persistent grid density (waves per CU; 0: the kernel's occupancy; "
+                         "< 0: occupancy less that many)")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="default run: seconds of each serving window (fixed rate, then the offered 2000 req/s)")
     ap.add_argument("--serve-rate", type=float, default=800.0,
@@ -207,7 +208,7 @@ def main():
     info = prob.info
     K = prob.num_tracks
     P = I * K
-    opts_mixed = hcx.default_opts(precision=hcx.PREC_MIXED if args.precision == "mixed" else hcx.PREC_FP64,
+    opts_head = hcx.default_opts(precision=hcx.PREC_MIXED if args.precision == "mixed" else hcx.PREC_FP64,
                                   lanes_per_path=args.lanes, waves_per_cu=args.waves_per_cu)
     opts_fp64 = hcx.default_opts(precision=hcx.PREC_FP64, lanes_per_path=args.lanes)
     batch = hcx.Batch(prob, I, target_params=p1)
@@ -236,13 +237,24 @@ def main():
             counts = hcx.collect(rj, opts, stream=sj)
         return counts
 
-    # ------------------------------------------------------------ mixed batched
-    log(f"bench: mixed batched, {I} instances x {K} paths, warmup {args.warmup}, steps {args.steps}, "
-        f"{nf} solves in flight")
-    run_steps(opts_mixed, args.warmup)
+    # ------------------------------------------------------------ batched (headline)
+    head = "mixed" if args.precision == "mixed" else "fp64"
+    other = "fp64" if head == "mixed" else "mixed"
+    opts_other = hcx.default_opts(precision=hcx.PREC_FP64 if other == "fp64" else hcx.PREC_MIXED,
+                                  lanes_per_path=args.lanes, waves_per_cu=args.waves_per_cu)
+    # clock ramp: untimed solves for about half a second, so that the timed
+    # steps do not start on an idle GPU's clocks (reported as prewarm_s)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 0.5:
+        run_steps(opts_head, nf)
+        torch.cuda.synchronize()
+    prewarm_s = time.perf_counter() - t0
+    log(f"bench: {head} batched, {I} instances x {K} paths, warmup {args.warmup}, steps {args.steps}, "
+        f"{nf} solves in flight (after {prewarm_s:.2f} s of untimed solves)")
+    run_steps(opts_head, args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    counts = run_steps(opts_mixed, args.steps)
+    counts = run_steps(opts_head, args.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
@@ -250,36 +262,33 @@ def main():
     h = res.to_host()
     lat_ms = (h["inst_done"][:I] - h["batch_start"][0]) / DEVICE_CLOCK_HZ * 1e3
     f32, f64 = flops_split(info, h, args.precision == "mixed")
-    log(f"bench: mixed {ms_per_step:.2f} ms/step, {value:.4g} converged paths/s, counts {counts}")
+    log(f"bench: {head} {ms_per_step:.2f} ms/step, {value:.4g} converged paths/s, counts {counts}")
 
-    # the same batched loop with the FP64 tracker (no FP32 stages, no
-    # duplicate re-track): reported beside the headline, not instead of it
-    fp64_batched = None
-    if args.precision == "mixed":
-        opts_b64 = hcx.default_opts(precision=hcx.PREC_FP64, lanes_per_path=args.lanes,
-                                    waves_per_cu=args.waves_per_cu)
-        run_steps(opts_b64, args.warmup)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        c64 = run_steps(opts_b64, args.steps)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        fp64_batched = {"value": c64["refined"] / (el / args.steps), "unit": "converged paths/s",
-                        "ms_per_step": el * 1e3 / args.steps, "counts": c64}
-        log(f"bench: fp64 batched {fp64_batched['ms_per_step']:.2f} ms/step, {fp64_batched['value']:.4g} paths/s")
+    # the same batched loop in the other precision: reported beside the
+    # headline, not instead of it
+    run_steps(opts_other, args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    c_o = run_steps(opts_other, args.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    other_batched = {"value": c_o["refined"] / (el / args.steps), "unit": "converged paths/s",
+                     "ms_per_step": el * 1e3 / args.steps, "counts": c_o,
+                     "counts_identical": bool(c_o == counts)}
+    log(f"bench: {other} batched {other_batched['ms_per_step']:.2f} ms/step, {other_batched['value']:.4g} paths/s")
 
     # kernel-only timing + FLOPs for the roofline (coefficient tables precomputed)
     hxs = np.stack([sysf.param_homotopy_coeffs(p0, p1[i])[0] for i in range(I)])
     hts = np.stack([sysf.param_homotopy_coeffs(p0, p1[i])[1] for i in range(I)])
     kbatch = hcx.Batch(prob, I, coeffs_hx=hxs, coeffs_ht=hts)
     kres = hcx.Result(P, prob.n, I)
-    hcx.track(prob, kbatch, opts_mixed, result=kres, stream=stream)
+    hcx.track(prob, kbatch, opts_head, result=kres, stream=stream)
     torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     reps = max(3, args.steps)
     ev[0].record(stream)
     for _ in range(reps):
-        hcx.track(prob, kbatch, opts_mixed, result=kres, stream=stream)
+        hcx.track(prob, kbatch, opts_head, result=kres, stream=stream)
     ev[1].record(stream)
     torch.cuda.synchronize()
     kernel_ms = ev[0].elapsed_time(ev[1]) / reps
@@ -329,9 +338,9 @@ def main():
                 "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
                 "launches": I, "seconds": el64}
         parity = {"counts_identical": bool(int(okm.sum()) == int(ok6.sum())),
-                  "refined_mixed": int(okm.sum()), "refined_fp64": int(ok6.sum()),
-                  "real_mixed": int(((h["flags"][:P] & hcx.FLAG_REAL) != 0).sum()),
-                  "real_fp64": int(((flags64 & hcx.FLAG_REAL) != 0).sum()),
+                  "refined_batched": int(okm.sum()), "refined_fp64_per_problem": int(ok6.sum()),
+                  "real_batched": int(((h["flags"][:P] & hcx.FLAG_REAL) != 0).sum()),
+                  "real_fp64_per_problem": int(((flags64 & hcx.FLAG_REAL) != 0).sum()),
                   "max_root_rel_err": float(err.max()) if len(err) else 0.0}
         log(f"bench: fp64 per-problem {v64:.4g} converged paths/s, p50 {fp64['p50_ms']:.3f} ms "
             f"p99 {fp64['p99_ms']:.3f} ms; parity {parity}")
@@ -360,7 +369,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": ("c64+c128 (FP32 RK4 predictor and Jacobian, FP64 corrector, duplicate re-track and refinement)"
-                  if args.precision == "mixed" else "c128"),
+                  if args.precision == "mixed" else
+                  "c128 (FP64 throughout: RK4 predictor, Jacobian, LU, Newton corrector, refinement)"),
         "data": "synthetic (seeded CN(0,1) parameters)",
         "config": {"workload": "n=6 parameter homotopy, 12 monomials/eq, k=24 params, 64 start roots, "
                                f"{I} target instances ({P} paths per step)",
@@ -377,8 +387,9 @@ def main():
         "counts": counts,
         "fp64_per_problem": fp64,
         "speedup_vs_fp64_per_problem": (value / fp64["value"]) if fp64 else None,
-        "fp64_batched": fp64_batched,
-        "parity_mixed_vs_fp64": parity,
+        f"{other}_batched": other_batched,
+        "prewarm_s": prewarm_s,
+        "parity_batched_vs_fp64_per_problem": parity,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
