@@ -488,7 +488,7 @@ def bench_config3(args):
     log("config3: seeding start roots of five types")
     types = workloads.quadratic_types(workloads.CONFIG3_TYPES, seed=args.seed)
     kinds = workloads.mixed_instances(len(types), I, args.seed)
-    opts = hcx.default_opts(precision=hcx.PREC_MIXED)
+    opts = hcx.default_opts(precision=_prec(args))
     probs, batches, tunes = [], [], []
     for ti, ty in enumerate(types):
         cnt = int((kinds == ti).sum())
@@ -530,7 +530,7 @@ def bench_config3(args):
     torch.cuda.synchronize()
     sep_ms = (time.perf_counter() - t0) * 1e3
     out = {"metric": METRIC, "value": conv / (ms * 1e-3), "unit": "converged paths/s", "ms_per_step": ms,
-           "steps": args.steps, "warmup": args.warmup, "dtype": "c64+c128 mixed",
+           "steps": args.steps, "warmup": args.warmup, "dtype": "c64+c128 mixed" if args.precision == "mixed" else "c128",
            "config": {"workload": f"config 3: uneven batch of 5 quadratic types (n,K)={workloads.CONFIG3_TYPES}, "
                                   f"{I} instances mixed uniformly, {paths} paths, one graph-captured launch",
                       "instances_per_type": [b.I for b in batches]},
@@ -736,7 +736,7 @@ def bench_config5(args):
     log("config5: seeding start roots of six types")
     types = workloads.quadratic_types(workloads.CONFIG5_TYPES, seed=args.seed)
     probs = [hcx.Problem.from_system(t["system"], t["start"], p0=t["p0"]) for t in types]
-    opts = hcx.default_opts(precision=hcx.PREC_MIXED)
+    opts = hcx.default_opts(precision=_prec(args))
     srv = hcx.Server(opts, hbm_budget_bytes=4 << 30, max_batch_requests=64, streams=4)
     tids = [srv.add_type(p) for p in probs]
     for ti, ty in enumerate(types):
@@ -750,7 +750,7 @@ def bench_config5(args):
     fp64 = _fp64_per_problem_mix(probs, types, per_type=24, seed=args.seed * 31)
     out = {"metric": METRIC, "value": saturated["converged_paths_per_s"], "unit": "converged paths/s",
            "config": {"workload": f"config 5: serving loop, Poisson arrivals, types (n,K)={workloads.CONFIG5_TYPES} "
-                                  f"drawn uniformly, one instance per request, mixed FP32/FP64",
+                                  f"drawn uniformly, one instance per request, {args.precision}",
                       "hbm_budget_bytes": 4 << 30, "streams": 4, "max_batch_requests": 64},
            "at_offered_rate": saturated, "at_fixed_rate": steady, "at_fixed_rate_high": steady_hi,
            "p50_ms": steady["p50_ms"], "p99_ms": steady["p99_ms"], "latency_rate_req_s": fixed,
@@ -794,7 +794,7 @@ def bench_type_sweep(args):
         del b, r
         prob.close()
     print(json.dumps({"metric": METRIC, "config": {"workload": f"per-type cost sweep, {B} requests per type, "
-                                                               "mixed FP32/FP64, one batched launch each"},
+                                                               f"{args.precision}, one batched launch each"},
                       "types": rows}))
 
 
