@@ -151,7 +151,7 @@ def cpu_baseline(sysf, p0, start, p1, seconds):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--instances", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=2)
