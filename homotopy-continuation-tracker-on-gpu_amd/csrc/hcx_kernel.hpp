@@ -1230,6 +1230,40 @@ void track_kernel(const Launch LN) {
   }
 }
 
+// One launch over segments of different sizes (uneven batches): segment s
+// runs the size-specialised loop of its n, n in {3, 5, 6, 9, 12} (terms of
+// degree <= DEG, compact tables in LDS, G = n), so a multi-type launch keeps
+// the specialised code of each type instead of the generic instance; the
+// kernel's registers and LDS are those of its largest member.
+template <int N, int DEG, int MODE>
+__device__ __forceinline__ void run_exact(const Launch& LN, int si, unsigned char* smem, int lane) {
+  if constexpr (MODE == 0) {
+    run_segment_fp64<N, DEG, true>(LN, si, smem, lane);
+  } else {
+    run_segment<N, DEG, MODE, true>(LN, si, smem, lane);
+  }
+}
+
+template <int DEG, int MODE>
+__global__ __launch_bounds__(256) void track_kernel_set(const Launch LN) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = (int)threadIdx.x & 63;
+  int s = 0;
+  while (s + 1 < LN.nseg && (int)blockIdx.x >= LN.first_block[s + 1]) ++s;
+  for (int tried = 0; tried < LN.nseg; ++tried) {
+    switch (LN.seg[s].n) {
+      case 3: run_exact<3, DEG, MODE>(LN, s, smem, lane); break;
+      case 5: run_exact<5, DEG, MODE>(LN, s, smem, lane); break;
+      case 6: run_exact<6, DEG, MODE>(LN, s, smem, lane); break;
+      case 9: run_exact<9, DEG, MODE>(LN, s, smem, lane); break;
+      case 12: run_exact<12, DEG, MODE>(LN, s, smem, lane); break;
+      default: break;
+    }
+    if (!LN.persistent) break;
+    s = s + 1 == LN.nseg ? 0 : s + 1;
+  }
+}
+
 using KernelFn = void (*)(const Launch);
 
 // Kernel instances live in separate translation units (compile in parallel).
@@ -1252,6 +1286,9 @@ HCX_GENERIC_UNITS(32, 1)
 #undef HCX_GENERIC_UNITS
 // Size-specialised instances (n = G = N) for the problem sizes of the
 // benchmark workloads, DEG 2 and 3.
+// Multi-size instances (n in {3, 5, 6, 9, 12}, DEG 2) for uneven batches.
+KernelFn kernel_set_m0();
+KernelFn kernel_set_m1();
 KernelFn kernel_x3_m0(int deg);
 KernelFn kernel_x3_m1(int deg);
 KernelFn kernel_x5_m0(int deg);

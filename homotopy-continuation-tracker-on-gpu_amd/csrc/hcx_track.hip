@@ -482,20 +482,30 @@ constexpr int kMaxTabLds = 32 * 1024;
 // Kernel instance for a launch: the size-specialised one when every segment
 // has the same n, runs its lanes per path (G = exact_lanes(n)) and fits
 // DEG <= 3 (and, for
-// n <= 16, has compact tables of at most kMaxTabLds bytes); otherwise the
-// generic instance with the smallest NMAX >= max n.
+// n <= 16, has compact tables of at most kMaxTabLds bytes); the multi-size
+// instance when the segments' sizes differ but all are in {3, 5, 6, 9, 12}
+// with G = n, quadratic terms and compact tables; otherwise the generic
+// instance with the smallest NMAX >= max n.
 hcx_status choose_kernel(int nseg, const hcx_problem* const* probs, const int* Gs, int max_deg, int mode,
                          int specialize, Plan& pl) {
   pl.mode = mode;
   pl.tab_lds = 0;
   int max_n = 1, tab = 0;
-  bool same = true, tab_ok = true;
+  bool same = true, tab_ok = true, in_set = true;
   for (int s = 0; s < nseg; ++s) {
     const int n = probs[s]->n;
     max_n = std::max(max_n, n);
     same = same && n == probs[0]->n && Gs[s] == exact_lanes(n);
+    in_set = in_set && (n == 3 || n == 5 || n == 6 || n == 9 || n == 12) && Gs[s] == n;
     tab_ok = tab_ok && probs[s]->d_ctab && probs[s]->ctab_bytes <= kMaxTabLds;
     tab = std::max(tab, probs[s]->ctab_bytes);
+  }
+  if (specialize && !same && in_set && tab_ok && max_deg <= 2) {
+    pl.fn = mode == 0 ? kernel_set_m0() : kernel_set_m1();
+    pl.nmax = max_n;
+    pl.exact = true;
+    pl.tab_lds = tab;
+    return HCX_OK;
   }
   const int n0 = probs[0]->n;
   if (specialize && same && (n0 > 16 || tab_ok)) {

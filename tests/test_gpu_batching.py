@@ -92,3 +92,28 @@ def test_uneven_multi_type_vs_oracle(hcx_mod, oracle_mod, types):
         ok = rg & ro
         err = np.abs(g["roots"][ok] - o["roots"][ok]).max(axis=1) / np.maximum(1, np.abs(o["roots"][ok]).max(axis=1))
         assert err.max() <= 1e-10, ty["n"]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_multi_size_launch_equals_single_type_launches(hcx_mod, types, prec):
+    """A launch over types of different sizes runs each segment's
+    size-specialised loop (the multi-size instance): flags, step and Newton
+    counts and roots equal those of one launch per type, bit for bit; the
+    generic instance (specialize = 0) gives the same counts."""
+    import torch
+    hcx = hcx_mod
+    probs, batches = make_batches(hcx, types, [9, 6, 4], 31)
+    opts = hcx.default_opts(precision=prec)
+    multi = [r.to_host() for r in hcx.track_multi(probs, batches, opts)]
+    torch.cuda.synchronize()
+    for p, b, m in zip(probs, batches, multi):
+        one = hcx.track(p, b, opts).to_host()
+        torch.cuda.synchronize()
+        for k in ("flags", "steps", "newton", "refine"):
+            assert (one[k] == m[k]).all(), (p.n, k)
+        both_nan = np.isnan(one["roots"]) & np.isnan(m["roots"])
+        assert ((one["roots"] == m["roots"]) | both_nan).all(), p.n
+    gen = [r.to_host() for r in hcx.track_multi(probs, batches, hcx.default_opts(precision=prec, specialize=0))]
+    torch.cuda.synchronize()
+    for g, m in zip(gen, multi):
+        assert (g["flags"] == m["flags"]).all() and (g["steps"] == m["steps"]).all()
