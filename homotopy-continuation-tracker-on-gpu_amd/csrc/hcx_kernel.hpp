@@ -490,13 +490,12 @@ struct Lds {
 // coefficients c(t) must already be in the group's LDS.  `run` enters as the
 // set of groups to correct; ok / inf return the reference's r_isSuccessful /
 // r_isInfFail.
-template <int NMAX, int GC, int DEG, class T, bool LT, class OT>
-__device__ __forceinline__ void newton_pass(const Seg& S, const OT& o, const Group& g, const Lds<NMAX>& L,
+template <int NMAX, int GC, int DEG, class T, bool LT, class OT, class SegT>
+__device__ __forceinline__ void newton_pass(const SegT& S, int n, const OT& o, const Group& g, const Lds<NMAX>& L,
                                             const uint2* lt, bool run, bool row, cx<double>& x, int& nnewt, bool& ok,
                                             bool& inf) {
   ok = false;
   inf = false;
-  const int n = S.n;
   const int gle = row ? g.gl : 0;
   cx<T>* sx = L.template x<T>();
   const cx<T>* sc = reinterpret_cast<const cx<T>*>(L.coef());
@@ -532,11 +531,10 @@ using namespace dev;
 // (x0 + dt k3, t + dt); returns k1 + 2 k2 + 2 k3 + k4 (component gl).
 // sd_ext: dc/dt table of the group's instance in global memory (constant in t,
 // M = 1) or nullptr to stage dc/dt(t) in LDS next to c(t).
-template <int NMAX, int GC, int DEG, class T, bool LT>
-__device__ __forceinline__ cx<double> predict(const Seg& S, const Group& g, const Lds<NMAX>& L, const uint2* lt,
+template <int NMAX, int GC, int DEG, class T, bool LT, class SegT>
+__device__ __forceinline__ cx<double> predict(const SegT& S, int n, const Group& g, const Lds<NMAX>& L, const uint2* lt,
                                               bool act, bool row, int gle, int inst, double ts, double dt,
                                               cx<double> xl, const cx<T>* sd_ext) {
-  const int n = S.n;
   const double h = 0.5 * dt;
   cx<double> acc = cmk<double>(0, 0), kprev = cmk<double>(0, 0);
   cx<T>* sx = L.template x<T>();
@@ -586,7 +584,9 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
   }
   // compile-time power-of-two lanes per path: DPP group reductions
   constexpr int GC = (EXACT && is_pow2(GE)) ? GE : 0;
-  const Opts o = LN.o;
+  const KOpts* const opts_arg = launch_opts();
+  const KSeg* const seg_arg = launch_seg(si);
+  int iter = 0;
   const Group g = make_group(S.G, lane);
   const int n = S.n;
   const bool row = g.in_grp && g.gl < n;
@@ -616,16 +616,16 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
   // lane-distributed state: component gl of x and of the last accepted x
   cx<double> x = cmk<double>(0, 0), xl = x;
 
-  auto start_path = [&](bool fp64) {
+  auto start_path = [&](const KSeg& Sv, bool fp64) {
     t = 0.0;
-    dt = o.dt0;
+    dt = opts_arg->dt0;
     cnt = 0;
     asteps = 0;
     ez = false;
     esc = fp64;
     status = 1;
     if (row) {
-      const double2 v = S.start[(size_t)root * S.start_stride + g.gl];
+      const double2 v = Sv.start[(size_t)root * Sv.start_stride + g.gl];
       x = cmk<double>(v.x, v.y);
     } else {
       x = cmk<double>(0, 0);
@@ -634,6 +634,9 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
   };
 
   for (;;) {
+    const KOpts& o = *iteration_view(opts_arg, iter);
+    const KSeg& Sk = *iteration_view(seg_arg, iter);
+    ++iter;
     // ------------------------------------------------ refill idle groups
     const bool want = status == 0 && !exhausted;
     if (__any(want)) {
@@ -641,9 +644,9 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
       if (LN.persistent) {
         unsigned int q = 0;
         if (want && g.gl == 0) {
-          q = atomicAdd(S.counter, 1u);
+          q = atomicAdd(Sk.counter, 1u);
           // path-list launch (FP64 re-track of selected paths)
-          if (S.plist) q = q < *S.plist_count ? (unsigned int)S.plist[q] : 0xffffffffu;
+          if (Sk.plist) q = q < *Sk.plist_count ? (unsigned int)Sk.plist[q] : 0xffffffffu;
         }
         p = (long long)(unsigned int)lane_read((int)q, g.gbase);
         if (p == 0xffffffffll) p = -1;
@@ -651,17 +654,17 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
         p = static_path;
       }
       if (want) {
-        if (p < 0 || p >= S.P) {
+        if (p < 0 || p >= Sk.P) {
           exhausted = true;
         } else {
           path = p;
-          inst = (int)(p / S.K);
-          root = (int)(p - (long long)inst * S.K);
+          inst = (int)(p / Sk.K);
+          root = (int)(p - (long long)inst * Sk.K);
           nsteps = 0;
           nnewt = 0;
           nref = 0;
-          fl = S.plist ? HCX_FLAG_ESCALATED : 0u;
-          start_path(MODE == 0);
+          fl = Sk.plist ? HCX_FLAG_ESCALATED : 0u;
+          start_path(Sk, MODE == 0);
           if (!LN.persistent) exhausted = true;
         }
       }
@@ -698,21 +701,21 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
         // dc/dt (constant when M = 1) from the instance's table in global
         // memory halves the group's LDS and raises occupancy; a launch of few
         // paths (latency bound) stages it in LDS instead
-        const cx<double>* sd_ext = (S.M == 1 && !LN.dot_lds)
-                                       ? reinterpret_cast<const cx<double>*>(S.cht + (size_t)inst * (S.C + 1))
+        const cx<double>* sd_ext = (Sk.M == 1 && !LN.dot_lds)
+                                       ? reinterpret_cast<const cx<double>*>(Sk.cht + (size_t)inst * (Sk.C + 1))
                                        : nullptr;
-        acc = predict<NMAX, GC, DEG, double, LT>(S, g, L, lt, stepping, row, gle, inst, ts, dt, xl, sd_ext);
+        acc = predict<NMAX, GC, DEG, double, LT>(Sk, n, g, L, lt, stepping, row, gle, inst, ts, dt, xl, sd_ext);
       } else if constexpr (!KESC) {
-        acc = predict<NMAX, GC, DEG, float, LT>(S, g, L, lt, stepping, row, gle, inst, ts, dt, xl, nullptr);
+        acc = predict<NMAX, GC, DEG, float, LT>(Sk, n, g, L, lt, stepping, row, gle, inst, ts, dt, xl, nullptr);
       } else {
         const bool p32 = stepping && !esc, p64 = stepping && esc;
         cx<double> a32 = cmk<double>(0, 0), a64 = a32;
-        if (__any(p32)) a32 = predict<NMAX, GC, DEG, float, LT>(S, g, L, lt, p32, row, gle, inst, ts, dt, xl, nullptr);
+        if (__any(p32)) a32 = predict<NMAX, GC, DEG, float, LT>(Sk, n, g, L, lt, p32, row, gle, inst, ts, dt, xl, nullptr);
         if (__any(p64)) {
           HCX_WAVE_SYNC();
           const cx<double>* sd_ext =
-              S.M == 1 ? reinterpret_cast<const cx<double>*>(S.cht + (size_t)inst * (S.C + 1)) : nullptr;
-          a64 = predict<NMAX, GC, DEG, double, LT>(S, g, L, lt, p64, row, gle, inst, ts, dt, xl, sd_ext);
+              Sk.M == 1 ? reinterpret_cast<const cx<double>*>(Sk.cht + (size_t)inst * (Sk.C + 1)) : nullptr;
+          a64 = predict<NMAX, GC, DEG, double, LT>(Sk, n, g, L, lt, p64, row, gle, inst, ts, dt, xl, sd_ext);
         }
         acc = esc ? a64 : a32;
       }
@@ -724,15 +727,15 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
       // Newton corrector at the new t (:189-223)
       bool ok = false, inf = false;
       if (MODE == 0) {
-        newton_pass<NMAX, GC, DEG, double, LT>(S, o, g, L, lt, stepping, row, x, nnewt, ok, inf);
+        newton_pass<NMAX, GC, DEG, double, LT>(Sk, n, o, g, L, lt, stepping, row, x, nnewt, ok, inf);
       } else {
         const bool run32 = stepping && !ez && !esc && o.newton32, run64 = stepping && !run32;
         bool ok32 = false, inf32 = false, ok64 = false, inf64 = false;
-        if (__any(run32)) newton_pass<NMAX, GC, DEG, float, LT>(S, o, g, L, lt, run32, row, x, nnewt, ok32, inf32);
+        if (__any(run32)) newton_pass<NMAX, GC, DEG, float, LT>(Sk, n, o, g, L, lt, run32, row, x, nnewt, ok32, inf32);
         if (__any(run64)) {
           HCX_WAVE_SYNC();
-          stage_coeffs<double>(S, g, run64, inst, t, L.coef(), false);
-          newton_pass<NMAX, GC, DEG, double, LT>(S, o, g, L, lt, run64, row, x, nnewt, ok64, inf64);
+          stage_coeffs<double>(Sk, g, run64, inst, t, L.coef(), false);
+          newton_pass<NMAX, GC, DEG, double, LT>(Sk, n, o, g, L, lt, run64, row, x, nnewt, ok64, inf64);
         }
         ok = run64 ? ok64 : ok32;
         inf = run64 ? inf64 : inf32;
@@ -779,7 +782,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
       const bool ref = fin && (fl & HCX_FLAG_TRACKED) && o.refine_iters > 0;
       double resid = 0.0;
       if (__any(ref)) {
-        stage_coeffs<double>(S, g, ref, inst, 1.0, L.coef(), false);
+        stage_coeffs<double>(Sk, g, ref, inst, 1.0, L.coef(), false);
         cx<double>* sx = L.template x<double>();
         const cx<double>* sc = reinterpret_cast<const cx<double>*>(L.coef());
         bool run = ref, conv = false, sing_any = false;
@@ -788,8 +791,8 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
           publish_x<double>(sx, x, g, n, run, row);
           HCX_WAVE_SYNC();
           cx<double> A[NMAX];
-          eval_jac_row<NMAX, DEG, double, LT>(S, n, gle, sx, sc, A, lt);
-          const cx<double> hv = eval_t_row<DEG, double, LT>(S, n, gle, sx, sc, lt);
+          eval_jac_row<NMAX, DEG, double, LT>(Sk, n, gle, sx, sc, A, lt);
+          const cx<double> hv = eval_t_row<DEG, double, LT>(Sk, n, gle, sx, sc, lt);
           bool sing;
           const cx<double> d = group_lu_solve<NMAX, GC, double>(A, hv, g, n, row, L.template piv<double>(), sing);
           if (run && row) x = csub(x, d);
@@ -810,7 +813,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
         // residual max_i |H_i(x, 1)|
         publish_x<double>(sx, x, g, n, ref, row);
         HCX_WAVE_SYNC();
-        const cx<double> hv = eval_t_row<DEG, double, LT>(S, n, gle, sx, sc, lt);
+        const cx<double> hv = eval_t_row<DEG, double, LT>(Sk, n, gle, sx, sc, lt);
         double r = row ? sqrt(cnorm2(hv)) : 0.0;
         if (r != r) r = INFINITY;
         resid = group_max(r, g);
@@ -832,9 +835,9 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
         nsteps = 0;
         nnewt = 0;
         nref = 0;
-        start_path(true);
+        start_path(Sk, true);
       } else if (fin) {
-        const SegOut* O = S.out;
+        const SegOut* O = Sk.out;
         double2* roots = O->roots;
         const int rstride = O->roots_stride;
         if (row) roots[(size_t)path * rstride + g.gl] = make_double2(x.re, x.im);
@@ -853,7 +856,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
           if (O->inst_left) {
             const int left = atomicSub(&O->inst_left[inst], 1);
             if (left == 1 && O->inst_done) O->inst_done[inst] = wall_clock64();
-          } else if (O->inst_done && S.plist) {
+          } else if (O->inst_done && Sk.plist) {
             atomicMax(&O->inst_done[inst], (unsigned long long)wall_clock64());
           }
         }
