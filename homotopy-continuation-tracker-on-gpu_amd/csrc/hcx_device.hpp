@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace hcx {
 namespace dev {
 
@@ -205,6 +207,17 @@ __device__ __forceinline__ int term_var(const Term& t, int q) {
 __device__ __forceinline__ int cterm_c(uint2 t) { return (int)(t.x & 0xffffu); }
 __device__ __forceinline__ int cterm_s(uint2 t) { return (int)t.x >> 16; }
 __device__ __forceinline__ int cterm_var(uint2 t, int q) { return (int)((t.y >> (8 * q)) & 0xffu); }
+
+// Compact 4-byte form for tables of degree <= 2 (each term has at most two
+// variables): coefficient slot (bits 0-11) | scalar (bits 12-15, two's
+// complement, -8..7) | variable 0 (bits 16-23) | variable 1 (bits 24-31).
+__device__ __forceinline__ int cterm_c(uint32_t t) { return (int)(t & 0xfffu); }
+__device__ __forceinline__ int cterm_s(uint32_t t) { return (int)(t << 16) >> 28; }
+__device__ __forceinline__ int cterm_var(uint32_t t, int q) { return (int)((t >> (16 + 8 * q)) & 0xffu); }
+
+// The compact term record of a table of degree DEG.
+template <int DEG>
+using CTerm = typename std::conditional<(DEG <= 2), uint32_t, uint2>::type;
 
 }  // namespace dev
 }  // namespace hcx

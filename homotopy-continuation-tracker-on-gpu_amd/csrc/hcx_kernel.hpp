@@ -81,7 +81,8 @@ struct Seg {
   const double2* cht;          // [I][C+1][M]
   const int* cslot;            // [C+1] LDS slot of coefficient j (the tables hold slots)
   const int* cinv;             // [C+1] coefficient of LDS slot q
-  const uint2* ctab;           // compact hx then ht tables (LDS-table instances)
+  const uint32_t* ctab;        // compact hx then ht tables (LDS-table instances; 4-byte terms
+                               // for DEG <= 2, else 8-byte)
   unsigned int* counter;
   const SegOut* out;           // device copy written by init_kernel
   const int* plist;            // optional: track only these paths (count in *plist_count)
@@ -145,17 +146,17 @@ __device__ __forceinline__ T* iteration_view(T* p, int iter) {
 // LT: the tables are the compact copy in LDS at lt (hx part first).
 template <int NMAX, int DEG, class T, bool LT, class SegT>
 __device__ __forceinline__ void eval_jac_row(const SegT& S, int n, int gle, const cx<T>* sx, const cx<T>* sc,
-                                             cx<T> (&A)[NMAX], const uint2* lt) {
+                                             cx<T> (&A)[NMAX], const void* lt) {
 #pragma unroll
   for (int j = 0; j < NMAX; ++j) {
     cx<T> acc = cmk<T>(0, 0);
     if (j < n) {
       const int cnt = S.hx_cnt[j];
       if constexpr (LT) {
-        const uint2* tp = lt + (size_t)j * S.hxT * n + gle;
+        const CTerm<DEG>* tp = static_cast<const CTerm<DEG>*>(lt) + (size_t)j * S.hxT * n + gle;
 #pragma unroll kTermUnroll<T>
         for (int k = 0; k < cnt; ++k) {
-          const uint2 tm = tp[(size_t)k * n];
+          const CTerm<DEG> tm = tp[(size_t)k * n];
           cx<T> v = cscale(sc[cterm_c(tm)], (T)cterm_s(tm));
 #pragma unroll
           for (int q = 0; q < DEG - 1; ++q) v = cmul(v, sx[cterm_var(tm, q)]);
@@ -192,13 +193,13 @@ __device__ __forceinline__ void eval_jac_row(const SegT& S, int n, int gle, cons
 // indexed by coefficient.
 template <int DEG, class T, bool LT, class SegT>
 __device__ __forceinline__ cx<T> eval_t_row(const SegT& S, int n, int gle, const cx<T>* sx, const cx<T>* coef,
-                                            const uint2* lt, const int* remap = nullptr) {
+                                            const void* lt, const int* remap = nullptr) {
   cx<T> acc = cmk<T>(0, 0);
   if constexpr (LT) {
-    const uint2* tp = lt + (size_t)n * S.hxT * n + gle;
+    const CTerm<DEG>* tp = static_cast<const CTerm<DEG>*>(lt) + (size_t)n * S.hxT * n + gle;
 #pragma unroll kTermUnroll<T>
     for (int k = 0; k < S.htT; ++k) {
-      const uint2 tm = tp[(size_t)k * n];
+      const CTerm<DEG> tm = tp[(size_t)k * n];
       const int c = cterm_c(tm);
       cx<T> v = cscale(coef[remap ? remap[c] : c], (T)cterm_s(tm));
 #pragma unroll
@@ -462,9 +463,10 @@ __device__ __forceinline__ cx<T> group_lu_solve(cx<T> (&A)[NMAX], cx<T> b, const
 // Every wave of the block takes part (one copy per block) and the block
 // barrier orders every wave's table reads after the copy; blocks of several
 // waves run one segment only, so each wave reaches the barrier once.
-__device__ __forceinline__ const uint2* stage_tables(const Seg& S, int n, unsigned char* smem) {
-  uint2* d = reinterpret_cast<uint2*>(smem);
-  const int words = (S.hxT * n + S.htT) * n;
+template <int DEG>
+__device__ __forceinline__ const void* stage_tables(const Seg& S, int n, unsigned char* smem) {
+  uint32_t* d = reinterpret_cast<uint32_t*>(smem);
+  const int words = (S.hxT * n + S.htT) * n * (int)(sizeof(CTerm<DEG>) / 4);
   for (int i = (int)threadIdx.x; i < words; i += (int)blockDim.x) d[i] = S.ctab[i];
   __syncthreads();
   return d;
@@ -492,7 +494,7 @@ struct Lds {
 // r_isInfFail.
 template <int NMAX, int GC, int DEG, class T, bool LT, class OT, class SegT>
 __device__ __forceinline__ void newton_pass(const SegT& S, int n, const OT& o, const Group& g, const Lds<NMAX>& L,
-                                            const uint2* lt, bool run, bool row, cx<double>& x, int& nnewt, bool& ok,
+                                            const void* lt, bool run, bool row, cx<double>& x, int& nnewt, bool& ok,
                                             bool& inf) {
   ok = false;
   inf = false;
@@ -532,7 +534,7 @@ using namespace dev;
 // sd_ext: dc/dt table of the group's instance in global memory (constant in t,
 // M = 1) or nullptr to stage dc/dt(t) in LDS next to c(t).
 template <int NMAX, int GC, int DEG, class T, bool LT, class SegT>
-__device__ __forceinline__ cx<double> predict(const SegT& S, int n, const Group& g, const Lds<NMAX>& L, const uint2* lt,
+__device__ __forceinline__ cx<double> predict(const SegT& S, int n, const Group& g, const Lds<NMAX>& L, const void* lt,
                                               bool act, bool row, int gle, int inst, double ts, double dt,
                                               cx<double> xl, const cx<T>* sd_ext) {
   const double h = 0.5 * dt;
@@ -591,13 +593,14 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
   const int n = S.n;
   const bool row = g.in_grp && g.gl < n;
   const int gle = row ? g.gl : 0;
-  // Size-specialised instances up to n = 16 read the segment's tables from a
-  // compact copy in LDS (at the start of the dynamic LDS, LN.tab_lds bytes)
-  // instead of global memory: the term loads head every evaluation's
-  // dependency chain, and LDS returns them several times sooner.
-  constexpr bool LT = EXACT && NMAX <= 16;
-  const uint2* lt = nullptr;
-  if constexpr (LT) lt = stage_tables(S, n, smem);
+  // Size-specialised instances up to n = 16, and n = 30 with quadratic
+  // terms, read the segment's tables from a compact copy in LDS (at the start
+  // of the dynamic LDS, LN.tab_lds bytes) instead of global memory: the term
+  // loads head every evaluation's dependency chain, and LDS returns them
+  // several times sooner.
+  constexpr bool LT = EXACT && (NMAX <= 16 || DEG <= 2);
+  const void* lt = nullptr;
+  if constexpr (LT) lt = stage_tables<DEG>(S, n, smem);
   // lanes outside the full groups (64 % G of them) alias region 0: they
   // track nothing and store nothing to LDS
   const int wv = (int)threadIdx.x >> 6;  // wave of the block
@@ -875,14 +878,14 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
 template <int DEG, bool LT, bool GDOT, class SegT>
 __device__ __forceinline__ cx<double> eval_rhs_sel(const SegT& S, int n, int gle, const cx<double>* sx,
                                                    const cx<double>* sc, const cx<double>* dtab, bool dot,
-                                                   const uint2* lt) {
+                                                   const void* lt) {
   const cx<double>* base = dot ? dtab : sc;
   cx<double> acc = cmk<double>(0, 0);
   if constexpr (LT) {
-    const uint2* tp = lt + (size_t)n * S.hxT * n + gle;
+    const CTerm<DEG>* tp = static_cast<const CTerm<DEG>*>(lt) + (size_t)n * S.hxT * n + gle;
 #pragma unroll kTermUnroll<double>
     for (int k = 0; k < S.htT; ++k) {
-      const uint2 tm = tp[(size_t)k * n];
+      const CTerm<DEG> tm = tp[(size_t)k * n];
       int c = cterm_c(tm);
       if (GDOT && dot) c = S.cinv[c];
       cx<double> v = cscale(base[c], (double)cterm_s(tm));
@@ -957,9 +960,9 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
   const int n = S.n;
   const bool row = g.in_grp && g.gl < n;
   const int gle = row ? g.gl : 0;
-  constexpr bool LT = EXACT && NMAX <= 16;
-  const uint2* lt = nullptr;
-  if constexpr (LT) lt = stage_tables(S, n, smem);
+  constexpr bool LT = EXACT && (NMAX <= 16 || DEG <= 2);
+  const void* lt = nullptr;
+  if constexpr (LT) lt = stage_tables<DEG>(S, n, smem);
   const int wv = (int)threadIdx.x >> 6;  // wave of the block
   Lds<NMAX> L{smem + LN.tab_lds + ((size_t)wv * g.gpw + (g.in_grp ? g.grp : 0)) * S.lds_group};
   cx<double>* sx = L.template x<double>();

@@ -23,8 +23,10 @@ struct hcx_problem {
   int* d_hx_cnt = nullptr;
   int* d_cslot = nullptr;              // [C+1] LDS slot of each coefficient
   int* d_cinv = nullptr;               // [C+1] coefficient of each slot
-  uint2* d_ctab = nullptr;             // compact hx + ht tables (8-byte terms), or null
+  uint32_t* d_ctab = nullptr;          // compact hx + ht tables (8-byte terms), or null
   int ctab_bytes = 0;                  // their size rounded up to 16 bytes
+  uint32_t* d_ctab4 = nullptr;         // the same with 4-byte terms (degree <= 2), or null
+  int ctab4_bytes = 0;
   double2* d_start = nullptr;
   double2* d_affine = nullptr;
   double2* d_p0 = nullptr;

@@ -470,12 +470,14 @@ int lds_group_bytes(int nmax, int C, int M, bool dot_lds = false) {
 struct Plan {
   int nmax = 0, mode = 0;
   bool exact = false;
+  bool tab4 = false; // the kernel reads 4-byte compact terms (DEG <= 2), else 8-byte
   int tab_lds = 0;   // LDS bytes of the compact tables (instances that hold them there)
   size_t lds = 0;
   KernelFn fn = nullptr;
 };
 
-// Largest compact table the size-specialised instances up to n = 16 stage in
+// Largest compact table the size-specialised instances (up to n = 16; n = 30
+// with quadratic terms) stage in
 // LDS; a problem with larger tables runs on the generic instance.
 constexpr int kMaxTabLds = 32 * 1024;
 
@@ -492,13 +494,18 @@ hcx_status choose_kernel(int nseg, const hcx_problem* const* probs, const int* G
   pl.tab_lds = 0;
   int max_n = 1, tab = 0;
   bool same = true, tab_ok = true, in_set = true;
+  // the compact term format of the kernel's degree: 4 bytes up to degree 2
+  const bool t4 = max_deg <= 2;
+  pl.tab4 = t4;
   for (int s = 0; s < nseg; ++s) {
     const int n = probs[s]->n;
     max_n = std::max(max_n, n);
     same = same && n == probs[0]->n && Gs[s] == exact_lanes(n);
     in_set = in_set && (n == 3 || n == 5 || n == 6 || n == 9 || n == 12) && Gs[s] == n;
-    tab_ok = tab_ok && probs[s]->d_ctab && probs[s]->ctab_bytes <= kMaxTabLds;
-    tab = std::max(tab, probs[s]->ctab_bytes);
+    const void* ct = t4 ? (const void*)probs[s]->d_ctab4 : (const void*)probs[s]->d_ctab;
+    const int cb = t4 ? probs[s]->ctab4_bytes : probs[s]->ctab_bytes;
+    tab_ok = tab_ok && ct && cb <= kMaxTabLds;
+    tab = std::max(tab, cb);
   }
   if (specialize && !same && in_set && tab_ok && max_deg <= 2) {
     pl.fn = mode == 0 ? kernel_set_m0() : kernel_set_m1();
@@ -508,13 +515,14 @@ hcx_status choose_kernel(int nseg, const hcx_problem* const* probs, const int* G
     return HCX_OK;
   }
   const int n0 = probs[0]->n;
-  if (specialize && same && (n0 > 16 || tab_ok)) {
+  // n = 30 holds the tables in LDS only with quadratic terms (4-byte records)
+  if (specialize && same && (tab_ok || (n0 > 16 && !t4))) {
     KernelFn fn = pick_exact(n0, max_deg, mode);
     if (fn) {
       pl.fn = fn;
       pl.nmax = n0;
       pl.exact = true;
-      pl.tab_lds = n0 <= 16 ? tab : 0;
+      pl.tab_lds = (n0 <= 16 || t4) ? tab : 0;
       return HCX_OK;
     }
   }
@@ -535,7 +543,7 @@ hcx_status choose_kernel(int nseg, const hcx_problem* const* probs, const int* G
 }
 
 // Fills the segment-independent part of a Seg from a problem.
-void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, int nmax) {
+void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, int nmax, bool tab4) {
   std::memset(&S, 0, sizeof(S));
   std::memset(&O, 0, sizeof(O));
   O.roots_stride = p->n;
@@ -552,7 +560,7 @@ void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, 
   S.hx_cnt = p->d_hx_cnt;
   S.cslot = p->d_cslot;
   S.cinv = p->d_cinv;
-  S.ctab = p->d_ctab;
+  S.ctab = tab4 ? p->d_ctab4 : p->d_ctab;
   S.start = p->d_start;
   S.start_stride = p->n;
 }
@@ -867,6 +875,20 @@ hcx_status hcx_problem_create(const hcx_problem_desc* d, hcx_problem** out) {
     for (const auto& tm : ht) ctab.push_back(pack(tm));
     if (!fits) ctab.clear();
   }
+  // compact 4-byte terms for degree <= 2: slot in 12 bits, scalar in 4 bits
+  // signed, two variables of 8 bits
+  std::vector<uint32_t> ctab4;
+  if (std::max(deg, 2) <= 2 && C <= 0xfff) {
+    bool fits = true;
+    auto pack4 = [&](const dev::Term& tm) {
+      if (tm.s < -8 || tm.s > 7) fits = false;
+      return ((uint32_t)tm.c & 0xfffu) | (((uint32_t)tm.s & 0xfu) << 12) | ((tm.v03 & 0xffffu) << 16);
+    };
+    ctab4.reserve(hx.size() + ht.size());
+    for (const auto& tm : hx) ctab4.push_back(pack4(tm));
+    for (const auto& tm : ht) ctab4.push_back(pack4(tm));
+    if (!fits) ctab4.clear();
+  }
 
   auto* p = new hcx_problem();
   (void)hipGetDevice(&p->device);
@@ -895,6 +917,13 @@ hcx_status hcx_problem_create(const hcx_problem_desc* d, hcx_problem** out) {
     return fail(e, "copy cslot");
   if ((e = hipMemcpy(p->d_cinv, cinv.data(), (C + 1) * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(e, "copy cinv");
+  if (!ctab4.empty()) {
+    p->ctab4_bytes = (int)((ctab4.size() * sizeof(uint32_t) + 15) / 16 * 16);
+    if ((e = hipMalloc(&p->d_ctab4, p->ctab4_bytes)) != hipSuccess) return fail(e, "hipMalloc ctab4");
+    if ((e = hipMemcpy(p->d_ctab4, ctab4.data(), ctab4.size() * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+      return fail(e, "copy ctab4");
+  }
   if (!ctab.empty()) {
     p->ctab_bytes = (int)((ctab.size() * sizeof(uint2) + 15) / 16 * 16);
     if ((e = hipMalloc(&p->d_ctab, p->ctab_bytes)) != hipSuccess) return fail(e, "hipMalloc ctab");
@@ -987,6 +1016,7 @@ hcx_status hcx_problem_destroy(hcx_problem* p) {
   if (p->d_cslot) (void)hipFree(p->d_cslot);
   if (p->d_cinv) (void)hipFree(p->d_cinv);
   if (p->d_ctab) (void)hipFree(p->d_ctab);
+  if (p->d_ctab4) (void)hipFree(p->d_ctab4);
   if (p->d_start) (void)hipFree(p->d_start);
   if (p->d_affine) (void)hipFree(p->d_affine);
   if (p->d_p0) (void)hipFree(p->d_p0);
@@ -1098,7 +1128,7 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
     const int G = Gs[s];
     Seg& S = LN.seg[s];
     SegOut& O = OP.out[s];
-    seg_from_problem(S, O, p, G, o->precision, nmax);
+    seg_from_problem(S, O, p, G, o->precision, nmax, pl.tab4);
     S.I = b.num_instances;
     S.P = (long long)b.num_instances * p->K;
     if (empty) {
@@ -1340,7 +1370,7 @@ double hcx_kernel_HC_Solver(void* stream, const hcx_problem* pc, int max_steps, 
   if (choose_kernel(1, &p, &G, p->deg, HCX_PREC_FP64, 1, pl) != HCX_OK) return -1.0;
   Seg& S = LN.seg[0];
   SegOut& O = OP.out[0];
-  seg_from_problem(S, O, p, G, HCX_PREC_FP64, pl.nmax);
+  seg_from_problem(S, O, p, G, HCX_PREC_FP64, pl.nmax, pl.tab4);
   S.start = reinterpret_cast<const double2*>(d_start_sols);
   S.start_stride = p->n + 1;
   O.roots = reinterpret_cast<double2*>(d_track);
@@ -1505,7 +1535,7 @@ hcx_status hcx_autotune(hcx_problem* p, const hcx_batch_desc* sample, const hcx_
       std::memset(&LN, 0, sizeof(LN));
       SegOut O;
       LN.nseg = 1;
-      seg_from_problem(LN.seg[0], O, p, G, o->precision, pl.nmax);
+      seg_from_problem(LN.seg[0], O, p, G, o->precision, pl.nmax, pl.tab4);
       if (finish_plan(LN, pl) == HCX_OK)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(pl.fn), 64, pl.lds);
     }
