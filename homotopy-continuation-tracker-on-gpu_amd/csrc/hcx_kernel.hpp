@@ -561,18 +561,19 @@ __device__ __forceinline__ cx<double> predict(const SegT& S, int n, const Group&
   return acc;
 }
 
-// Lane-group path loop.  MODE 0: everything FP64.  MODE 1 (mixed): FP32 RK4
+// Mixed-precision lane-group path loop (MODE 1; the FP64 mode runs
+// run_segment_fp64), phase-synchronous: FP32 RK4
 // predictor (Jacobian, dH/dt and LU of the four stages), FP64 Newton
 // corrector (FP32 before the end zone when o.newton32), FP64 refinement; a
 // path that ends without a refined root is re-tracked from t = 0 entirely in
 // FP64 (escalation): by the same group (KESC: the generic instances and
 // n = 30), or, for the size-specialised instances up to n = 16, by the FP64
-// pass after the launch, which keeps the FP64 predictor out of the kernel:
-// its register footprint then allows one more wave per SIMD (n = 6: 214 ->
-// 166 VGPRs, n = 12: 260 -> 241).  The pass skips paths already escalated,
-// so either way each failed path is re-tracked once.
+// pass after the launch, which keeps the FP64 predictor out of the kernel and
+// with it the registers of a second predictor.  The pass skips paths already
+// escalated, so either way each failed path is re-tracked once.
 template <int NMAX, int DEG, int MODE, bool EXACT>
 __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned char* smem, int lane) {
+  static_assert(MODE == 1, "the FP64 mode runs run_segment_fp64");
   // local copies: the fields are wave-uniform and live in SGPRs (scalar
   // kernarg loads) instead of being re-read through a generic pointer
   Seg S = LN.seg[si];
@@ -611,7 +612,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
   int inst = 0, root = 0;
   double t = 0.0, dt = 0.0;
   int cnt = 0, nsteps = 0, asteps = 0, nnewt = 0, nref = 0;
-  bool ez = false, esc = (MODE == 0);  // this path runs in FP64
+  bool ez = false, esc = false;  // this path runs in FP64
   int status = 0;  // 0 idle, 1 tracking, 2 finishing
   uint32_t fl = 0;
   bool exhausted = !g.in_grp;
@@ -667,7 +668,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
           nnewt = 0;
           nref = 0;
           fl = Sk.plist ? HCX_FLAG_ESCALATED : 0u;
-          start_path(Sk, MODE == 0);
+          start_path(Sk, false);
           if (!LN.persistent) exhausted = true;
         }
       }
@@ -700,15 +701,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
       }
       const double ts = t;
       cx<double> acc;
-      if constexpr (MODE == 0) {
-        // dc/dt (constant when M = 1) from the instance's table in global
-        // memory halves the group's LDS and raises occupancy; a launch of few
-        // paths (latency bound) stages it in LDS instead
-        const cx<double>* sd_ext = (Sk.M == 1 && !LN.dot_lds)
-                                       ? reinterpret_cast<const cx<double>*>(Sk.cht + (size_t)inst * (Sk.C + 1))
-                                       : nullptr;
-        acc = predict<NMAX, GC, DEG, double, LT>(Sk, n, g, L, lt, stepping, row, gle, inst, ts, dt, xl, sd_ext);
-      } else if constexpr (!KESC) {
+      if constexpr (!KESC) {
         acc = predict<NMAX, GC, DEG, float, LT>(Sk, n, g, L, lt, stepping, row, gle, inst, ts, dt, xl, nullptr);
       } else {
         const bool p32 = stepping && !esc, p64 = stepping && esc;
@@ -729,9 +722,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
       }
       // Newton corrector at the new t (:189-223)
       bool ok = false, inf = false;
-      if (MODE == 0) {
-        newton_pass<NMAX, GC, DEG, double, LT>(Sk, n, o, g, L, lt, stepping, row, x, nnewt, ok, inf);
-      } else {
+      {
         const bool run32 = stepping && !ez && !esc && o.newton32, run64 = stepping && !run32;
         bool ok32 = false, inf32 = false, ok64 = false, inf64 = false;
         if (__any(run32)) newton_pass<NMAX, GC, DEG, float, LT>(Sk, n, o, g, L, lt, run32, row, x, nnewt, ok32, inf32);
@@ -830,7 +821,7 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
         }
         HCX_WAVE_SYNC();
       }
-      if (MODE == 1 && KESC && fin && !esc && !(fl & HCX_FLAG_REFINED)) {
+      if (KESC && fin && !esc && !(fl & HCX_FLAG_REFINED)) {
         // precision escalation: the mixed attempt did not reach a refined
         // root; track the same path again from t = 0 in FP64 (the counters
         // then describe the FP64 attempt, as after the re-track pass)
