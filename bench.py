@@ -177,8 +177,11 @@ def main():
                     help="end with one 256 MiB device copy, the reference dispatch for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scale factors")
     ap.add_argument("--config", type=int, default=2,
-                    help="workload: 2 (default, the headline line), 1, 3, 4, 5, or 6 (per-type cost "
-                         "sweep of the serving types)")
+                    help="workload: 2 (default, the headline line), 1, 3, 4, 5, 6 (per-type cost "
+                         "sweep of the serving types) or 7 (the reference's own problems, one instance "
+                         "per launch, beside the reference's published GPU times)")
+    ap.add_argument("--problems", default="",
+                    help="config 7: comma-separated problem names (default: all with a published time)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -186,7 +189,7 @@ def main():
         sys.exit(2)
     if args.config != 2:
         return {1: bench_config1, 3: bench_config3, 4: bench_config4, 5: bench_config5,
-                6: bench_type_sweep}[args.config](args)
+                6: bench_type_sweep, 7: bench_reference_problems}[args.config](args)
 
     import numpy as np
     import torch
@@ -796,6 +799,93 @@ def bench_type_sweep(args):
     print(json.dumps({"metric": METRIC, "config": {"workload": f"per-type cost sweep, {B} requests per type, "
                                                                f"{args.precision}, one batched launch each"},
                       "types": rows}))
+
+
+
+# The reference's published GPU-HC times (README.md:45-63: one launch tracking
+# all paths of one problem instance, complex FP32, GPU not stated), for the
+# problems whose committed fixture holds the full start-solution set.
+REFERENCE_README_MS = {
+    "uncalibrated_trifocal_rel_pos_CY": ("trifocal relative pose, unknown focal length", 668, 16.67),
+    "generalized_3views_4pts": ("generalized 3-view relative pose from 4 points", 583, 4.77),
+    "5pt_rel_pos_full_form_quat": ("5-point relative pose with depths", 40, 7.26),
+    "optimal_PnP_quat": ("optimal PnP (quaternion)", 80, 0.77),
+    "dual_reciever_TDOA_5pt": ("dual-receiver TDOA 5-points", 24, 3.17),
+    "distorted_2view_triangulation": ("distorted 2-view triangulation", 28, 0.83),
+    "5pt_rel_pos_alg_form_quat": ("5-point relative pose (reduced/algebraic form)", 40, 1.77),
+    "optimal_P4P_abs_pos": ("optimal P4P absolute pose", 32, 0.43),
+    "6pt_RS_abs_pos": ("6-point rolling shutter absolute pose", 20, 0.65),
+    "3pt_rel_pos_homog": ("3-point relative pose with homography constraint", 8, 0.84),
+    "PnP_unkn_principal_pt": ("PnP with unknown principal point", 8, 3.45),
+    "rel_pos_quiver": ("relative pose using quiver, unknown focal length", 20, 0.75),
+    "P3P": ("P3P, absolute pose", 8, 0.15),
+}
+
+
+def bench_reference_problems(args):
+    """The reference's own problems (tables, coefficient maps, start
+    solutions and settings from the committed fixtures), one instance per
+    launch in the reference's launch shape (static grid, one solve at a
+    time, synchronous), FP64, timed with HIP events around P2C + tracker;
+    beside each, the reference's published GPU-HC time for the same problem
+    and path count (complex FP32, GPU not stated)."""
+    import numpy as np
+    import torch
+    import hcx
+    fix = os.path.join(ROOT, "tests", "golden", "reference")
+    stream = torch.cuda.current_stream()
+    reps = max(5, args.steps)
+    rows = []
+    only = set(filter(None, args.problems.split(",")))
+    for name, (title, paths, ref_ms) in REFERENCE_README_MS.items():
+        if only and name not in only:
+            continue
+        d = np.load(os.path.join(fix, name + ".npz"), allow_pickle=False)
+        start = d["start_sols"]
+        if start.shape[0] != paths:
+            continue
+        if str(d["kind"]) != "P2C":
+            continue
+        n = int(d["n"])
+        hx4 = d["hx"].reshape(n, n, int(d["hx_terms"]), int(d["hx_parts"]))
+        ht3 = d["ht"].reshape(n, int(d["ht_terms"]), int(d["ht_parts"]))
+        prob = hcx.Problem(n, int(d["C"]), hx4, ht3, start, max_order_t=int(d["M"]),
+                           cmap=(d["cmap_scalar"], d["cmap_params"]), p0=d["start_params"])
+        k = int(d["k"])
+        tp = d["target_params"] if bool(d["target_complete"]) else hcx.random_cn(args.seed + 1, k)
+        b = hcx.Batch(prob, 1, target_params=tp.reshape(1, k))
+        out = {}
+        for label, static, prec in (("fp64_per_problem", True, hcx.PREC_FP64),
+                                    ("fp64_persistent", False, hcx.PREC_FP64)):
+            o = hcx.default_opts(precision=prec, max_steps=int(d["max_steps"]), max_corr=int(d["max_corr"]),
+                                 incr_steps=int(d["incr_steps"]))
+            r = hcx.track(prob, b, o, static=static, stream=stream)
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(reps):
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                ev[0].record(stream)
+                hcx.track(prob, b, o, result=r, static=static, stream=stream)
+                ev[1].record(stream)
+                torch.cuda.synchronize()
+                ts.append(ev[0].elapsed_time(ev[1]))
+            h = r.to_host()
+            out[label] = {"ms_p50": percentile(ts, 50), "ms_min": float(min(ts)),
+                          "tracked": int(((h["flags"] & hcx.FLAG_TRACKED) != 0).sum()),
+                          "refined": int(((h["flags"] & hcx.FLAG_REFINED) != 0).sum()),
+                          "steps_max": int(h["steps"].max())}
+        row = {"problem": name, "title": title, "n": int(d["n"]), "paths": paths,
+               "reference_gpu_hc_ms": ref_ms, **out,
+               "speedup_vs_reference": ref_ms / out["fp64_per_problem"]["ms_p50"]}
+        rows.append(row)
+        log(f"reference problem {name}: n={row['n']} paths={paths}: {out['fp64_per_problem']['ms_p50']:.3f} ms "
+            f"(persistent {out['fp64_persistent']['ms_p50']:.3f} ms) vs reference {ref_ms} ms")
+        del b
+        prob.close()
+    print(json.dumps({"metric": METRIC, "config": {"workload": "the reference's problems, one instance per "
+                                                               "launch, reference settings, FP64"},
+                      "problems": rows,
+                      "note": "reference times: README.md:45-63, complex FP32 on an unstated GPU"}))
 
 
 if __name__ == "__main__":

@@ -154,22 +154,35 @@ __device__ __forceinline__ double dpp_read(double v) {
 #define HCX_WAVE_SYNC() __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront")
 
 // Group geometry of one lane.
+// A path's lanes: R replicas of G lanes each (R = 1 but in launches of few
+// paths).  The replicas hold identical path state and perform identical
+// operations, except that each sums every R-th term of the evaluation's
+// table rows; a butterfly over the replicas then gives all of them the same
+// full sums.
 struct Group {
-  int G;      // lanes per path
-  int gpw;    // full groups per wave
+  int G;      // lanes per replica (rows / equations)
+  int R;      // replicas per path
+  int gpw;    // full groups (paths) per wave
   int grp;    // group index of this lane
-  int gl;     // lane index inside the group (row / equation)
-  int gbase;  // wave lane of the group's lane 0
+  int s;      // replica index of this lane
+  int gl;     // lane index inside the replica (row / equation)
+  int gbase;  // wave lane of the replica's lane 0
+  int wbase;  // wave lane of the group's lane 0 (replica 0's lane 0)
   bool in_grp;// lane belongs to a full group
 };
 
-__device__ __forceinline__ Group make_group(int G, int lane) {
+__device__ __forceinline__ Group make_group(int G, int lane, int R = 1) {
   Group g;
+  const int W = G * R;
   g.G = G;
-  g.gpw = 64 / G;
-  g.grp = lane / G;
-  g.gl = lane - g.grp * G;
-  g.gbase = g.grp * G;
+  g.R = R;
+  g.gpw = 64 / W;
+  g.grp = lane / W;
+  const int w = lane - g.grp * W;
+  g.s = R > 1 ? w / G : 0;
+  g.gl = w - g.s * G;
+  g.wbase = g.grp * W;
+  g.gbase = g.wbase + g.s * G;
   g.in_grp = g.grp < g.gpw;
   return g;
 }

@@ -170,6 +170,7 @@ void hcx_track_opts_default(hcx_track_opts* o) {
   o->retrack_duplicates = 1;
   o->fp32_corrector = 0;
   o->real_ref_tol = 1e-4;
+  o->replicas = 0;
 }
 
 hcx_status hcx_random_cn(uint64_t seed, int64_t count, hcx_complex* out) {

@@ -69,6 +69,7 @@ struct Seg {
   int n, C, M, K;
   int hxT, htT;
   int G;
+  int R;                       // replicas per path (lanes per path = G * R; FP64 loop)
   int lds_group;               // bytes of one group's LDS region
   int start_stride;
   int I;
@@ -142,11 +143,46 @@ __device__ __forceinline__ T* iteration_view(T* p, int iter) {
 }
 
 // ------------------------------------------------------------ evaluation
+// sum_k s_k * coef[c_k] * prod_{d < NV} x[var_kd] over the cnt terms of one
+// table row (records n apart from tp, full Term records in global memory or
+// in an LDS copy).  The records of a chunk of terms are all requested before
+// the first is used, so their latencies overlap; remap (when not null) maps
+// a record's coefficient number to its index in coef.  Terms k0, k0 + ks,
+// k0 + 2 ks, ... only (the replica's share of the row).
+template <int NV, class T>
+__device__ __forceinline__ cx<T> term_row_sum(const Term* tp, int cnt, int n, const cx<T>* sx, const cx<T>* coef,
+                                              const int* remap, int k0 = 0, int ks = 1) {
+  cx<T> acc = cmk<T>(0, 0);
+  for (int kb = k0; kb < cnt; kb += kTermChunk * ks) {
+    Term tm[kTermChunk];
+    int ci[kTermChunk];
+#pragma unroll
+    for (int q = 0; q < kTermChunk; ++q)
+      if (kb + q * ks < cnt) tm[q] = tp[(size_t)(kb + q * ks) * n];
+#pragma unroll
+    for (int q = 0; q < kTermChunk; ++q)
+      if (kb + q * ks < cnt) ci[q] = remap ? remap[tm[q].c] : tm[q].c;
+#pragma unroll
+    for (int q = 0; q < kTermChunk; ++q) {
+      if (kb + q * ks < cnt) {
+        cx<T> v = cscale(coef[ci[q]], (T)tm[q].s);
+#pragma unroll
+        for (int d = 0; d < NV; ++d) v = cmul(v, sx[term_var(tm[q], d)]);
+        acc = cadd(acc, v);
+      }
+    }
+  }
+  return acc;
+}
+
 // Row gl of dH/dx (dev-eval-indxing-generalized_3views_4pts.cuh:70-86).
-// LT: the tables are the compact copy in LDS at lt (hx part first).
+// LT: the tables are the compact copy in LDS at lt (hx part first); else
+// full Term records, from the LDS copy at lt when tl, else global memory.
+// Terms k0, k0 + ks, ... of each entry only (a replica's share).
 template <int NMAX, int DEG, class T, bool LT, class SegT>
 __device__ __forceinline__ void eval_jac_row(const SegT& S, int n, int gle, const cx<T>* sx, const cx<T>* sc,
-                                             cx<T> (&A)[NMAX], const void* lt) {
+                                             cx<T> (&A)[NMAX], const void* lt, bool tl = false, int k0 = 0,
+                                             int ks = 1) {
 #pragma unroll
   for (int j = 0; j < NMAX; ++j) {
     cx<T> acc = cmk<T>(0, 0);
@@ -155,7 +191,7 @@ __device__ __forceinline__ void eval_jac_row(const SegT& S, int n, int gle, cons
       if constexpr (LT) {
         const CTerm<DEG>* tp = static_cast<const CTerm<DEG>*>(lt) + (size_t)j * S.hxT * n + gle;
 #pragma unroll kTermUnroll<T>
-        for (int k = 0; k < cnt; ++k) {
+        for (int k = k0; k < cnt; k += ks) {
           const CTerm<DEG> tm = tp[(size_t)k * n];
           cx<T> v = cscale(sc[cterm_c(tm)], (T)cterm_s(tm));
 #pragma unroll
@@ -163,24 +199,11 @@ __device__ __forceinline__ void eval_jac_row(const SegT& S, int n, int gle, cons
           acc = cadd(acc, v);
         }
       } else {
-        // tables in global memory: the records of a chunk of terms are all
-        // requested before the first is used, so their latencies overlap
-        const Term* tp = S.hx + (size_t)j * S.hxT * n + gle;
-        for (int k0 = 0; k0 < cnt; k0 += kTermChunk) {
-          Term tm[kTermChunk];
-#pragma unroll
-          for (int q = 0; q < kTermChunk; ++q)
-            if (k0 + q < cnt) tm[q] = tp[(size_t)(k0 + q) * n];
-#pragma unroll
-          for (int q = 0; q < kTermChunk; ++q) {
-            if (k0 + q < cnt) {
-              cx<T> v = cscale(sc[tm[q].c], (T)tm[q].s);
-#pragma unroll
-              for (int d = 0; d < DEG - 1; ++d) v = cmul(v, sx[term_var(tm[q], d)]);
-              acc = cadd(acc, v);
-            }
-          }
-        }
+        const size_t off = (size_t)j * S.hxT * n + gle;
+        if (tl)
+          acc = term_row_sum<DEG - 1, T>(static_cast<const Term*>(lt) + off, cnt, n, sx, sc, nullptr, k0, ks);
+        else
+          acc = term_row_sum<DEG - 1, T>(S.hx + off, cnt, n, sx, sc, nullptr, k0, ks);
       }
     }
     A[j] = acc;
@@ -233,13 +256,59 @@ __device__ __forceinline__ cx<T> eval_t_row(const SegT& S, int n, int gle, const
   return acc;
 }
 
+// c_j(t) and dc_j/dt(t) of order MC in t into LDS slot cslot[j] (Horner
+// from the highest term, as the general loop of stage_coeffs): the terms of
+// c_j are requested before the first Horner step uses one, then those of
+// dc_j/dt, so two memory latencies per coefficient instead of one per term.
+template <int MC, class T, class SegT>
+__device__ __forceinline__ void stage_coeffs_m(const SegT& S, const Group& g, const double2* hx, const double2* ht,
+                                               double t, cx<T>* sc, cx<T>* sd, bool with_dot) {
+  const int C1 = S.C + 1;
+#pragma unroll 1
+  for (int j = g.gl + g.s * g.G; j < C1; j += g.G * g.R) {
+    const int q = S.cslot[j];
+    {
+      const double2* ha = hx + (size_t)j * (MC + 1);
+      double2 a[MC + 1];
+#pragma unroll
+      for (int m = 0; m <= MC; ++m) a[m] = ha[m];
+      double cr = a[MC].x, ci = a[MC].y;
+#pragma unroll
+      for (int m = MC - 1; m >= 0; --m) {
+        const double nr = cr * t + a[m].x, ni = ci * t + a[m].y;
+        cr = nr;
+        ci = ni;
+      }
+      sc[q] = cmk<T>((T)cr, (T)ci);
+    }
+    if (with_dot) {
+      const double2* hb = ht + (size_t)j * MC;
+      double2 b[MC];
+#pragma unroll
+      for (int m = 0; m < MC; ++m) b[m] = hb[m];
+      double dr = b[MC - 1].x, di = b[MC - 1].y;
+#pragma unroll
+      for (int m = MC - 2; m >= 0; --m) {
+        const double nr = dr * t + b[m].x, ni = di * t + b[m].y;
+        dr = nr;
+        di = ni;
+      }
+      sd[q] = cmk<T>((T)dr, (T)di);
+    }
+  }
+}
+
 // Coefficient polynomials in t of the group's instance into LDS
 // (eval_parameter_homotopy, …cuh:34-67), computed in FP64 and stored as T.
 // Coefficient j goes to slot cslot[j]: slots follow the order in which the
 // rows of a group read them (term by term), so the lanes of a group that read
 // term k of their rows together hit consecutive LDS words instead of words one
 // equation apart (a bank conflict per lane pair when equations have many terms).
-template <class T, class SegT>
+// The replicas of a group share its LDS region and stage every R-th
+// coefficient each.  HOIST: orders M <= 4 request all terms of a coefficient
+// before its Horner steps (stage_coeffs_m; the FP64 loop, whose register
+// budget has room for them).
+template <class T, bool HOIST = false, class SegT>
 __device__ __forceinline__ void stage_coeffs(const SegT& S, const Group& g, bool act, int inst, double t,
                                              unsigned char* scb, bool with_dot) {
   if (!act) return;
@@ -251,7 +320,7 @@ __device__ __forceinline__ void stage_coeffs(const SegT& S, const Group& g, bool
   if (M == 1) {
     // c(t) = c0 + t c1, dc/dt = Ht[j][0]
 #pragma unroll kTermUnroll<T>
-    for (int j = g.gl; j < C1; j += g.G) {
+    for (int j = g.gl + g.s * g.G; j < C1; j += g.G * g.R) {
       const int q = S.cslot[j];
       const double2 a = hx[2 * (size_t)j], b = hx[2 * (size_t)j + 1];
       sc[q] = cmk<T>((T)(a.x + t * b.x), (T)(a.y + t * b.y));
@@ -262,7 +331,15 @@ __device__ __forceinline__ void stage_coeffs(const SegT& S, const Group& g, bool
     }
     return;
   }
-  for (int j = g.gl; j < C1; j += g.G) {
+  if constexpr (HOIST) {
+    switch (M) {
+      case 2: return stage_coeffs_m<2, T>(S, g, hx, ht, t, sc, sd, with_dot);
+      case 3: return stage_coeffs_m<3, T>(S, g, hx, ht, t, sc, sd, with_dot);
+      case 4: return stage_coeffs_m<4, T>(S, g, hx, ht, t, sc, sd, with_dot);
+      default: break;
+    }
+  }
+  for (int j = g.gl + g.s * g.G; j < C1; j += g.G * g.R) {
     double2 a = hx[(size_t)j * (M + 1) + M];
     double cr = a.x, ci = a.y;
     for (int m = M - 1; m >= 0; --m) {
@@ -468,6 +545,22 @@ __device__ __forceinline__ const void* stage_tables(const Seg& S, int n, unsigne
   uint32_t* d = reinterpret_cast<uint32_t*>(smem);
   const int words = (S.hxT * n + S.htT) * n * (int)(sizeof(CTerm<DEG>) / 4);
   for (int i = (int)threadIdx.x; i < words; i += (int)blockDim.x) d[i] = S.ctab[i];
+  __syncthreads();
+  return d;
+}
+
+// The full Term records of the generic instances (dH/dx part, then dH/dt
+// part, in their global layout) copied into LDS by the block's waves; for
+// launches of few paths (latency bound: no other waves hide the global
+// latency of the evaluation's record loads).  Same barrier rule as
+// stage_tables.
+__device__ __forceinline__ const void* stage_terms(const Seg& S, int n, unsigned char* smem) {
+  uint4* d = reinterpret_cast<uint4*>(smem);
+  const int nhx = S.hxT * n * n, nht = S.htT * n;
+  const uint4* hx = reinterpret_cast<const uint4*>(S.hx);
+  const uint4* ht = reinterpret_cast<const uint4*>(S.ht);
+  for (int i = (int)threadIdx.x; i < nhx; i += (int)blockDim.x) d[i] = hx[i];
+  for (int i = (int)threadIdx.x; i < nht; i += (int)blockDim.x) d[nhx + i] = ht[i];
   __syncthreads();
   return d;
 }
@@ -869,13 +962,13 @@ __device__ __forceinline__ void run_segment(const Launch& LN, int si, unsigned c
 template <int DEG, bool LT, bool GDOT, class SegT>
 __device__ __forceinline__ cx<double> eval_rhs_sel(const SegT& S, int n, int gle, const cx<double>* sx,
                                                    const cx<double>* sc, const cx<double>* dtab, bool dot,
-                                                   const void* lt) {
+                                                   const void* lt, bool tl = false, int k0 = 0, int ks = 1) {
   const cx<double>* base = dot ? dtab : sc;
   cx<double> acc = cmk<double>(0, 0);
   if constexpr (LT) {
     const CTerm<DEG>* tp = static_cast<const CTerm<DEG>*>(lt) + (size_t)n * S.hxT * n + gle;
 #pragma unroll kTermUnroll<double>
-    for (int k = 0; k < S.htT; ++k) {
+    for (int k = k0; k < S.htT; k += ks) {
       const CTerm<DEG> tm = tp[(size_t)k * n];
       int c = cterm_c(tm);
       if (GDOT && dot) c = S.cinv[c];
@@ -885,26 +978,12 @@ __device__ __forceinline__ cx<double> eval_rhs_sel(const SegT& S, int n, int gle
       acc = cadd(acc, v);
     }
   } else {
-    const Term* tp = S.ht + gle;
-    for (int k0 = 0; k0 < S.htT; k0 += kTermChunk) {
-      Term tm[kTermChunk];
-      int ci[kTermChunk];
-#pragma unroll
-      for (int q = 0; q < kTermChunk; ++q)
-        if (k0 + q < S.htT) tm[q] = tp[(size_t)(k0 + q) * n];
-#pragma unroll
-      for (int q = 0; q < kTermChunk; ++q)
-        if (k0 + q < S.htT) ci[q] = (GDOT && dot) ? S.cinv[tm[q].c] : tm[q].c;
-#pragma unroll
-      for (int q = 0; q < kTermChunk; ++q) {
-        if (k0 + q < S.htT) {
-          cx<double> v = cscale(base[ci[q]], (double)tm[q].s);
-#pragma unroll
-          for (int d = 0; d < DEG; ++d) v = cmul(v, sx[term_var(tm[q], d)]);
-          acc = cadd(acc, v);
-        }
-      }
-    }
+    const int* remap = (GDOT && dot) ? S.cinv : nullptr;
+    if (tl)
+      acc = term_row_sum<DEG, double>(static_cast<const Term*>(lt) + (size_t)n * S.hxT * n + gle, S.htT, n, sx, base,
+                                      remap, k0, ks);
+    else
+      acc = term_row_sum<DEG, double>(S.ht + gle, S.htT, n, sx, base, remap, k0, ks);
   }
   return acc;
 }
@@ -935,7 +1014,12 @@ enum : int {
 // of the groups of a wave changes.  c(t) is staged in LDS only when the
 // group's evaluation point moves (RK4 stages 0, 1 and 3; the corrector
 // reuses stage 3's c(t + dt)).
-template <int NMAX, int DEG, bool EXACT>
+// REP: the segment runs R > 1 replicas per path (Seg::R; launches of few
+// paths), compiled apart so that the loop of R = 1 carries none of it.
+// HOIST: stage_coeffs requests all terms of a coefficient at once (orders
+// M <= 4; up to n = 16, outside the multi-size kernel, whose registers
+// would otherwise cost it occupancy).
+template <int NMAX, int DEG, bool EXACT, bool REP, bool HOIST = (NMAX <= 16)>
 __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsigned char* smem, int lane) {
   Seg S = LN.seg[si];
   constexpr int GE = exact_lanes(NMAX);
@@ -947,13 +1031,24 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
   const KOpts* const opts_arg = launch_opts();
   const KSeg* const seg_arg = launch_seg(si);
   int iter = 0;
-  const Group g = make_group(S.G, lane);
+  const Group g = make_group(S.G, lane, REP ? S.R : 1);
+  const int k0 = REP ? g.s : 0, ks = REP ? g.R : 1;  // the replica's terms of each sum
   const int n = S.n;
   const bool row = g.in_grp && g.gl < n;
   const int gle = row ? g.gl : 0;
   constexpr bool LT = EXACT && (NMAX <= 16 || DEG <= 2);
   const void* lt = nullptr;
-  if constexpr (LT) lt = stage_tables<DEG>(S, n, smem);
+  // tl: the generic instance's full Term records are in LDS (launches of
+  // few paths; tab_lds then holds their bytes)
+  bool tl = false;
+  if constexpr (LT) {
+    lt = stage_tables<DEG>(S, n, smem);
+  } else if (!EXACT) {
+    if (LN.tab_lds > 0) {
+      lt = stage_terms(S, n, smem);
+      tl = true;
+    }
+  }
   const int wv = (int)threadIdx.x >> 6;  // wave of the block
   Lds<NMAX> L{smem + LN.tab_lds + ((size_t)wv * g.gpw + (g.in_grp ? g.grp : 0)) * S.lds_group};
   cx<double>* sx = L.template x<double>();
@@ -984,11 +1079,11 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
       long long p;
       if (LN.persistent) {
         unsigned int q = 0;
-        if (want && g.gl == 0) {
+        if (want && g.gl == 0 && g.s == 0) {
           q = atomicAdd(Sk.counter, 1u);
           if (Sk.plist) q = q < *Sk.plist_count ? (unsigned int)Sk.plist[q] : 0xffffffffu;
         }
-        p = (long long)(unsigned int)lane_read((int)q, g.gbase);
+        p = (long long)(unsigned int)lane_read((int)q, g.wbase);
         if (p == 0xffffffffll) p = -1;
       } else {
         p = static_path;
@@ -1072,18 +1167,36 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
       const double te = stg ? ts + a : (pe == kPhNewton ? t : 1.0);
       const cx<double> xe = stg ? cmk<double>(xl.re + a * kp.re, xl.im + a * kp.im) : x;
       const bool restage = act && te != tst;
-      if (__any(restage)) stage_coeffs<double>(Sk, g, restage, inst, te, L.coef(), !gdot);
+      if (__any(restage)) stage_coeffs<double, HOIST>(Sk, g, restage, inst, te, L.coef(), !gdot);
       if (restage) tst = te;
       publish_x<double>(sx, xe, g, n, act, row);
       HCX_WAVE_SYNC();
       cx<double> A[NMAX];
-      eval_jac_row<NMAX, DEG, double, LT>(Sk, n, gle, sx, sc, A, lt);
+      eval_jac_row<NMAX, DEG, double, LT>(Sk, n, gle, sx, sc, A, lt, tl, k0, ks);
       cx<double> r;
       if (gdot) {
         const cx<double>* dg = reinterpret_cast<const cx<double>*>(Sk.cht + (size_t)inst * (Sk.C + 1));
-        r = eval_rhs_sel<DEG, LT, true>(Sk, n, gle, sx, sc, dg, stg, lt);
+        r = eval_rhs_sel<DEG, LT, true>(Sk, n, gle, sx, sc, dg, stg, lt, tl, k0, ks);
       } else {
-        r = eval_rhs_sel<DEG, LT, false>(Sk, n, gle, sx, sc, sc + (Sk.C + 1), stg, lt);
+        r = eval_rhs_sel<DEG, LT, false>(Sk, n, gle, sx, sc, sc + (Sk.C + 1), stg, lt, tl, k0, ks);
+      }
+      if constexpr (REP) {
+        // each replica summed every R-th term of the row's sums: a butterfly
+        // over the replicas (all sums of a level exchanged together) leaves
+        // the full sums, identical in every replica (each level adds a pair
+        // of partial sums, commutatively)
+        for (int m = 1; m < g.R; m <<= 1) {
+          const int src = (lane + ((g.s ^ m) - g.s) * g.G) & 63;
+#pragma unroll
+          for (int j = 0; j < NMAX; ++j) {
+            if (j < n) {
+              A[j].re += __shfl(A[j].re, src);
+              A[j].im += __shfl(A[j].im, src);
+            }
+          }
+          r.re += __shfl(r.re, src);
+          r.im += __shfl(r.im, src);
+        }
       }
       bool sing;
       const cx<double> d =
@@ -1180,8 +1293,9 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
       const SegOut* O = Sk.out;
       double2* roots = O->roots;
       const int rstride = O->roots_stride;
-      if (row) roots[(size_t)path * rstride + g.gl] = make_double2(x.re, x.im);
-      if (g.gl == 0) {
+      // replica 0 stores (the other replicas hold the same values)
+      if (row && g.s == 0) roots[(size_t)path * rstride + g.gl] = make_double2(x.re, x.im);
+      if (g.gl == 0 && g.s == 0) {
         if (rstride > n) roots[(size_t)path * rstride + n] = make_double2(1.0, 0.0);
         if (O->flags) O->flags[path] = fl;
         if (O->steps) O->steps[path] = nsteps;
@@ -1218,13 +1332,23 @@ void track_kernel(const Launch LN) {
   while (s + 1 < LN.nseg && (int)blockIdx.x >= LN.first_block[s + 1]) ++s;
   for (int tried = 0; tried < LN.nseg; ++tried) {
     if constexpr (MODE == 0) {
-      run_segment_fp64<NMAX, DEG, EXACT>(LN, s, smem, lane);
+      run_segment_fp64<NMAX, DEG, EXACT, false>(LN, s, smem, lane);
     } else {
       run_segment<NMAX, DEG, MODE, EXACT>(LN, s, smem, lane);
     }
     if (!LN.persistent) break;
     s = s + 1 == LN.nseg ? 0 : s + 1;
   }
+}
+
+// FP64 launches of one segment with R > 1 replicas per path (launches of
+// few paths, latency bound: one wave per SIMD at most, so the register
+// budget is a full SIMD's).
+template <int NMAX, int DEG, bool EXACT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
+void track_kernel_rep(const Launch LN) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  run_segment_fp64<NMAX, DEG, EXACT, true>(LN, 0, smem, (int)threadIdx.x & 63);
 }
 
 // One launch over segments of different sizes (uneven batches): segment s
@@ -1235,7 +1359,7 @@ void track_kernel(const Launch LN) {
 template <int N, int DEG, int MODE>
 __device__ __forceinline__ void run_exact(const Launch& LN, int si, unsigned char* smem, int lane) {
   if constexpr (MODE == 0) {
-    run_segment_fp64<N, DEG, true>(LN, si, smem, lane);
+    run_segment_fp64<N, DEG, true, false, false>(LN, si, smem, lane);
   } else {
     run_segment<N, DEG, MODE, true>(LN, si, smem, lane);
   }
@@ -1300,5 +1424,13 @@ KernelFn kernel_x16_m0(int deg);
 KernelFn kernel_x16_m1(int deg);
 KernelFn kernel_x30_m0(int deg);
 KernelFn kernel_x30_m1(int deg);
+// Replica instances (FP64, one segment, R > 1): size-specialised for
+// n in {3, 5, 6} and {9, 12, 16} (DEG 2 and 3), generic for NMAX 8, 16 and 32
+// (DEG 2, 3 and 6).
+KernelFn kernel_rep_xs(int n, int deg);
+KernelFn kernel_rep_xl(int n, int deg);
+KernelFn kernel_rep_g8(int deg);
+KernelFn kernel_rep_g16(int deg);
+KernelFn kernel_rep_g32(int deg);
 
 }  // namespace hcx

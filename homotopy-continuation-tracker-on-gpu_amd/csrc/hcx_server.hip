@@ -380,6 +380,10 @@ hcx_status hcx_server_create(const hcx_server_opts* so, const hcx_track_opts* to
   auto* s = new hcx_server();
   s->so = *so;
   s->to = *to;
+  // no replica groups: a request's launch shape (alone or in a mix of
+  // types) then never changes its term sums, so its results are the same
+  // bit for bit whatever else is on the GPU
+  s->to.replicas = 1;
   hipError_t e = hipMalloc(&s->d_arena, (size_t)so->hbm_budget_bytes);
   if (e != hipSuccess) {
     hcx::set_error("hcx_server_create: cannot allocate the %lld-byte budget: %s", (long long)so->hbm_budget_bytes,

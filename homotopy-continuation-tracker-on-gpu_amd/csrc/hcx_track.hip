@@ -314,6 +314,23 @@ static KernelFn pick_kernel(int nmax, int deg, int mode) {
   }
 }
 
+// The replica instance of a launch planned on `pl` (nullptr: none).
+static KernelFn pick_rep(int nmax, int deg, bool exact) {
+  if (exact) {
+    if (deg > 3) return nullptr;
+    deg = deg < 2 ? 2 : deg;
+    if (nmax == 3 || nmax == 5 || nmax == 6) return kernel_rep_xs(nmax, deg);
+    if (nmax == 9 || nmax == 12 || nmax == 16) return kernel_rep_xl(nmax, deg);
+    return nullptr;
+  }
+  switch (nmax) {
+    case 8: return kernel_rep_g8(deg);
+    case 16: return kernel_rep_g16(deg);
+    case 32: return kernel_rep_g32(deg);
+    default: return nullptr;
+  }
+}
+
 static KernelFn pick_exact(int n, int deg, int mode) {
   if (deg > 3) return nullptr;
   deg = deg < 2 ? 2 : deg;
@@ -424,7 +441,8 @@ hcx_status validate_opts(const hcx_track_opts* o) {
   if (o->max_steps < 0 || o->max_corr < 1 || o->incr_steps < 1 || !(o->dt0 > 0) || !(o->end_zone >= 0) ||
       !(o->t_tol > 0) || o->refine_iters < 0 || (o->precision != HCX_PREC_FP64 && o->precision != HCX_PREC_MIXED) ||
       o->lanes_per_path < 0 || o->lanes_per_path > 64 || o->specialize < 0 || o->specialize > 1 ||
-      o->waves_per_cu < -64 || o->waves_per_cu > 64) {
+      o->waves_per_cu < -64 || o->waves_per_cu > 64 || o->replicas < 0 || o->replicas > 64 ||
+      (o->replicas & (o->replicas - 1)) != 0) {
     set_error("invalid tracker options (max_steps=%d max_corr=%d incr=%d precision=%d lanes=%d)", o->max_steps,
               o->max_corr, o->incr_steps, o->precision, o->lanes_per_path);
     return HCX_ERR_INVALID_ARG;
@@ -469,6 +487,7 @@ int lds_group_bytes(int nmax, int C, int M, bool dot_lds = false) {
 
 struct Plan {
   int nmax = 0, mode = 0;
+  int deg = 2;       // term degree of the instance (generic: rounded up to its instance's)
   bool exact = false;
   bool tab4 = false; // the kernel reads 4-byte compact terms (DEG <= 2), else 8-byte
   int tab_lds = 0;   // LDS bytes of the compact tables (instances that hold them there)
@@ -480,6 +499,8 @@ struct Plan {
 // with quadratic terms) stage in
 // LDS; a problem with larger tables runs on the generic instance.
 constexpr int kMaxTabLds = 32 * 1024;
+// LDS per workgroup on gfx950
+constexpr int kMaxLdsBytes = 160 * 1024;
 
 // Kernel instance for a launch: the size-specialised one when every segment
 // has the same n, runs its lanes per path (G = exact_lanes(n)) and fits
@@ -510,6 +531,7 @@ hcx_status choose_kernel(int nseg, const hcx_problem* const* probs, const int* G
   if (specialize && !same && in_set && tab_ok && max_deg <= 2) {
     pl.fn = mode == 0 ? kernel_set_m0() : kernel_set_m1();
     pl.nmax = max_n;
+    pl.deg = 2;
     pl.exact = true;
     pl.tab_lds = tab;
     return HCX_OK;
@@ -521,6 +543,7 @@ hcx_status choose_kernel(int nseg, const hcx_problem* const* probs, const int* G
     if (fn) {
       pl.fn = fn;
       pl.nmax = n0;
+      pl.deg = max_deg;
       pl.exact = true;
       pl.tab_lds = (n0 <= 16 || t4) ? tab : 0;
       return HCX_OK;
@@ -534,6 +557,7 @@ hcx_status choose_kernel(int nseg, const hcx_problem* const* probs, const int* G
   }
   pl.fn = pick_kernel(kNmaxList[ni], kDegList[di], mode);
   pl.nmax = kNmaxList[ni];
+  pl.deg = kDegList[di];
   pl.exact = false;
   if (!pl.fn) {
     set_error("no kernel instance for n=%d degree=%d mode=%d", max_n, max_deg, mode);
@@ -554,6 +578,7 @@ void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, 
   S.hxT = p->hxT;
   S.htT = p->htT;
   S.G = G;
+  S.R = 1;
   S.lds_group = lds_group_bytes(nmax, p->C, p->M);
   S.hx = p->d_hx;
   S.ht = p->d_ht;
@@ -565,19 +590,28 @@ void seg_from_problem(Seg& S, SegOut& O, const hcx_problem* p, int G, int mode, 
   S.start_stride = p->n;
 }
 
-hcx_status finish_plan(Launch& LN, Plan& pl) {
-  if (LN.wpb < 1) LN.wpb = 1;
+// Dynamic LDS bytes of a launch: the tables (when staged) and one region per
+// full group of every wave of the block; the lanes of a partial group share
+// their wave's region 0 and never write LDS.
+size_t plan_lds(const Launch& LN, const Plan& pl) {
   size_t lds = 0;
   for (int s = 0; s < LN.nseg; ++s) {
     const Seg& S = LN.seg[s];
-    // one region per full group of every wave of the block; the lanes of a
-    // partial group share their wave's region 0 and never write LDS
-    const int groups = 64 / S.G;
-    lds = std::max(lds, (size_t)LN.wpb * (size_t)groups * (size_t)S.lds_group);
+    const int groups = 64 / (S.G * S.R);
+    lds = std::max(lds, (size_t)std::max(LN.wpb, 1) * (size_t)groups * (size_t)S.lds_group);
   }
+  return lds + (size_t)pl.tab_lds;
+}
+
+hcx_status finish_plan(Launch& LN, Plan& pl) {
+  if (LN.wpb < 1) LN.wpb = 1;
+  const size_t lds = plan_lds(LN, pl);
   LN.tab_lds = pl.tab_lds;
-  lds += (size_t)pl.tab_lds;
   pl.lds = lds;
+  if (lds > (size_t)kMaxLdsBytes) {
+    set_error("launch needs %zu bytes of LDS per block (at most %d)", lds, kMaxLdsBytes);
+    return HCX_ERR_UNSUPPORTED;
+  }
   if (lds > 65536) {
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(pl.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
@@ -597,9 +631,9 @@ hcx_status choose_waves_per_block(Launch& LN, Plan& pl, int& best_wpb, int& best
   for (int wpb : {1, 2, 4}) {
     LN.wpb = wpb;
     Plan q = pl;
+    if (wpb > 1 && plan_lds(LN, q) > (size_t)kMaxLdsBytes) continue;
     hcx_status st = finish_plan(LN, q);
     if (st != HCX_OK) return st;
-    if (q.lds > (size_t)160 * 1024) continue;
     int nb = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(pl.fn), 64 * wpb,
                                                          q.lds));
@@ -1187,6 +1221,50 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
     const int gpw = 64 / G;
     seg_groups[s] = (S.P + gpw - 1) / gpw;
     total_groups_needed += seg_groups[s];
+  }
+  // A one-segment FP64 launch of at most one wave per SIMD is latency bound:
+  // no other waves hide the latency of its evaluations.  dc/dt then goes to
+  // LDS next to c(t), the generic instances read their Term records from an
+  // LDS copy, and each path gets R replicas of its lane group (the most, a
+  // power of two with R * G <= 64 and at least 4 dH/dt terms per replica,
+  // that keep the launch within one wave per SIMD; or o->replicas), which
+  // split the evaluation's term sums, when the group regions and the records
+  // fit in 64 KiB.
+  if (o->precision == HCX_PREC_FP64 && nseg == 1 && (total_groups_needed <= 4LL * g_cu || o->replicas > 1)) {
+    Seg& S = LN.seg[0];
+    const hcx_problem* p = probs[0];
+    const int lg = lds_group_bytes(nmax, p->C, p->M, true);
+    const size_t terms =
+        pl.exact ? 0 : ((size_t)p->n * p->n * p->hxT + (size_t)p->n * p->htT) * sizeof(dev::Term);
+    auto waves = [&](int R) {
+      const long long gpw = 64 / (S.G * R);
+      return (S.P + gpw - 1) / gpw;
+    };
+    int R = 1;
+    if (o->replicas > 0) {
+      R = o->replicas;
+      if (R * S.G > 64) {
+        set_error("track: %d replicas of %d lanes exceed a wave", R, S.G);
+        return HCX_ERR_INVALID_ARG;
+      }
+    } else {
+      // every replica keeps at least 4 terms of the row's dH/dt sum (the
+      // longest single sum): below that the butterfly costs more than the
+      // split saves
+      while (2 * R * S.G <= 64 && 8 * R <= p->htT && waves(2 * R) <= 4LL * g_cu) R *= 2;
+    }
+    const KernelFn rf = R > 1 ? pick_rep(pl.nmax, pl.deg, pl.exact) : nullptr;
+    if (!rf) R = 1;
+    const size_t regions = (size_t)(64 / (S.G * R)) * (size_t)lg;
+    if (regions + (size_t)pl.tab_lds + terms <= 65536) {
+      LN.dot_lds = 1;
+      S.lds_group = lg;
+      S.R = R;
+      if (R > 1) pl.fn = rf;
+      if (!pl.exact) pl.tab_lds = (int)terms;
+      seg_groups[0] = waves(R);
+      total_groups_needed = seg_groups[0];
+    }
   }
   LN.wpb = 1;
   int occ_blocks = -1;  // blocks per CU at full occupancy, when already known

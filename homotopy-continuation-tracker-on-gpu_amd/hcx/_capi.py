@@ -18,7 +18,7 @@ class TrackOpts(C.Structure):
         ("refine_iters", C.c_int), ("refine_tol", C.c_double), ("real_tol", C.c_double),
         ("unique_tol", C.c_double), ("precision", C.c_int), ("lanes_per_path", C.c_int),
         ("waves_per_cu", C.c_int), ("specialize", C.c_int), ("retrack_duplicates", C.c_int), ("fp32_corrector", C.c_int),
-        ("real_ref_tol", C.c_double),
+        ("real_ref_tol", C.c_double), ("replicas", C.c_int),
     ]
 
 

@@ -106,8 +106,11 @@ def test_multi_size_launch_equals_single_type_launches(hcx_mod, types, prec):
     opts = hcx.default_opts(precision=prec)
     multi = [r.to_host() for r in hcx.track_multi(probs, batches, opts)]
     torch.cuda.synchronize()
+    # one segment of few paths may split its term sums over replica groups
+    # (different rounding); replicas = 1 keeps the sums of the multi-size launch
+    one_opts = hcx.default_opts(precision=prec, replicas=1)
     for p, b, m in zip(probs, batches, multi):
-        one = hcx.track(p, b, opts).to_host()
+        one = hcx.track(p, b, one_opts).to_host()
         torch.cuda.synchronize()
         for k in ("flags", "steps", "newton", "refine"):
             assert (one[k] == m[k]).all(), (p.n, k)

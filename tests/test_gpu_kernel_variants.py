@@ -23,6 +23,9 @@ def _type(hcx, n, K, seed=5):
 
 def _run(hcx, prob, batch, **kw):
     import torch
+    # whole term sums per lane unless a test asks for replica groups (their
+    # split sums round differently)
+    kw.setdefault("replicas", 1)
     r = hcx.track(prob, batch, hcx.default_opts(**kw))
     torch.cuda.synchronize()
     return r.to_host()
@@ -76,4 +79,44 @@ def test_n30_dpp_groups(hcx_mod, gpu, prec):
     print(f"[n=30 G=30 prec={prec}] refined {int(ra.sum())}, agree {agree} / {int(both.sum())}", flush=True)
     assert both.sum() >= 0.99 * ra.sum()
     assert agree >= 0.99 * both.sum()
+    prob.close()
+
+
+@pytest.mark.parametrize("replicas", [2, 4, 8])
+def test_replica_groups_match_whole_sums(hcx_mod, gpu, oracle_mod, replicas):
+    """FP64 launches of few paths split every term sum of the evaluation over
+    R replica groups of each path (each replica sums every R-th term, a
+    butterfly adds the partial sums).  Only the summation order changes: the
+    flags and step counts of every path equal those of the launch with whole
+    sums per lane (replicas = 1), and the refined roots agree to 1e-10
+    relative, as they do with the oracle's."""
+    import torch
+    from hcx import workloads
+    hcx, oracle = hcx_mod, oracle_mod
+    ty = workloads.quadratic_types([(6, 64)], seed=9)[0]
+    prob = hcx.Problem.from_system(ty["system"], ty["start"], p0=ty["p0"])
+    b = hcx.Batch(prob, 2, target_params=workloads.targets(ty["k"], 2, seed=78))
+    one = _run(hcx, prob, b, precision=0, replicas=1)
+    rep = _run(hcx, prob, b, precision=0, replicas=replicas)
+    assert (one["flags"] == rep["flags"]).all()
+    assert (one["steps"] == rep["steps"]).all()
+    ok = (one["flags"] & hcx.FLAG_REFINED) != 0
+    assert ok.mean() > 0.9
+    err = np.abs(one["roots"][ok] - rep["roots"][ok]).max(axis=1) / np.maximum(
+        1.0, np.abs(one["roots"][ok]).max(axis=1))
+    print(f"[replicas={replicas}] refined {int(ok.sum())}, max rel diff {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-10
+    # and against the oracle on the same paths
+    sysf = ty["system"]
+    hxi, hti = sysf.tables()
+    op = oracle.OracleProblem(6, sysf.info["num_coeffs"], hxi, hti, 1)
+    ohx, oht = oracle.p2c_affine(sysf.affine(), ty["p0"], b.target_params.cpu().numpy())
+    o = op.track(ty["start"], ohx, oht, threads=8)
+    ro = (o["flags"] & hcx.FLAG_REFINED) != 0
+    assert (ro == ((rep["flags"] & hcx.FLAG_REFINED) != 0)).all()
+    both = ro & ok
+    e2 = np.abs(rep["roots"][both] - o["roots"][both]).max(axis=1) / np.maximum(
+        1.0, np.abs(o["roots"][both]).max(axis=1))
+    assert e2.max() <= 1e-10
+    torch.cuda.synchronize()
     prob.close()

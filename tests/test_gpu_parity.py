@@ -195,8 +195,11 @@ def test_multi_segment_same_size(hcx_mod, gpu, prec):
     opts = hcx.default_opts(precision=prec)
     multi = hcx.track_multi(probs, batches, opts)
     torch.cuda.synchronize()
+    # replicas = 1: the separate launch keeps whole term sums per lane, as
+    # the multi-segment launch does
+    one_opts = hcx.default_opts(precision=prec, replicas=1)
     for prob, b, m in zip(probs, batches, multi):
-        s = hcx.track(prob, b, opts).to_host()
+        s = hcx.track(prob, b, one_opts).to_host()
         mh = m.to_host()
         torch.cuda.synchronize()
         assert (mh["flags"] == s["flags"]).all()
