@@ -298,6 +298,57 @@ __device__ __forceinline__ void stage_coeffs_m(const SegT& S, const Group& g, co
   }
 }
 
+// c_j(t) and dc_j/dt(t) of order MC in t (MC >= 1) into LDS slot cslot[j]
+// for Q of the lane's coefficients at a time: every term of the Q
+// coefficients (both tables) is requested before the first Horner step, so
+// one memory latency per Q coefficients (the latency instances, whose
+// register budget is a whole SIMD's).  Surplus entries of the last chunk
+// reload coefficient C and store nothing.
+template <int MC, int Q, class T, class SegT>
+__device__ __forceinline__ void stage_coeffs_q(const SegT& S, const Group& g, const double2* hx, const double2* ht,
+                                               double t, cx<T>* sc, cx<T>* sd, bool with_dot) {
+  const int C1 = S.C + 1, st = g.G * g.R;
+#pragma unroll 1
+  for (int jb = g.gl + g.s * g.G; jb < C1; jb += Q * st) {
+    double2 a[Q][MC + 1], b[Q][MC];
+    int q[Q];
+#pragma unroll
+    for (int u = 0; u < Q; ++u) {
+      const int j = min(jb + u * st, C1 - 1);
+      q[u] = S.cslot[j];
+#pragma unroll
+      for (int m = 0; m <= MC; ++m) a[u][m] = hx[(size_t)j * (MC + 1) + m];
+      if (with_dot) {
+#pragma unroll
+        for (int m = 0; m < MC; ++m) b[u][m] = ht[(size_t)j * MC + m];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < Q; ++u) {
+      if (jb + u * st < C1) {
+        double cr = a[u][MC].x, ci = a[u][MC].y;
+#pragma unroll
+        for (int m = MC - 1; m >= 0; --m) {
+          const double nr = cr * t + a[u][m].x, ni = ci * t + a[u][m].y;
+          cr = nr;
+          ci = ni;
+        }
+        sc[q[u]] = cmk<T>((T)cr, (T)ci);
+        if (with_dot) {
+          double dr = b[u][MC - 1].x, di = b[u][MC - 1].y;
+#pragma unroll
+          for (int m = MC - 2; m >= 0; --m) {
+            const double nr = dr * t + b[u][m].x, ni = di * t + b[u][m].y;
+            dr = nr;
+            di = ni;
+          }
+          sd[q[u]] = cmk<T>((T)dr, (T)di);
+        }
+      }
+    }
+  }
+}
+
 // Coefficient polynomials in t of the group's instance into LDS
 // (eval_parameter_homotopy, …cuh:34-67), computed in FP64 and stored as T.
 // Coefficient j goes to slot cslot[j]: slots follow the order in which the
@@ -307,8 +358,9 @@ __device__ __forceinline__ void stage_coeffs_m(const SegT& S, const Group& g, co
 // The replicas of a group share its LDS region and stage every R-th
 // coefficient each.  HOIST: orders M <= 4 request all terms of a coefficient
 // before its Horner steps (stage_coeffs_m; the FP64 loop, whose register
-// budget has room for them).
-template <class T, bool HOIST = false, class SegT>
+// budget has room for them).  CH > 0: orders M <= 4 stage CH coefficients
+// per lane at a time (stage_coeffs_q; the latency instances).
+template <class T, bool HOIST = false, int CH = 0, class SegT>
 __device__ __forceinline__ void stage_coeffs(const SegT& S, const Group& g, bool act, int inst, double t,
                                              unsigned char* scb, bool with_dot) {
   if (!act) return;
@@ -317,6 +369,16 @@ __device__ __forceinline__ void stage_coeffs(const SegT& S, const Group& g, bool
   const int C1 = S.C + 1, M = S.M;
   const double2* hx = S.chx + (size_t)inst * C1 * (M + 1);
   const double2* ht = S.cht + (size_t)inst * C1 * M;
+  if constexpr (CH > 0) {
+    // chunks of CH coefficients up to order 2, CH / 2 above (registers)
+    switch (M) {
+      case 1: return stage_coeffs_q<1, CH, T>(S, g, hx, ht, t, sc, sd, with_dot);
+      case 2: return stage_coeffs_q<2, CH, T>(S, g, hx, ht, t, sc, sd, with_dot);
+      case 3: return stage_coeffs_q<3, (CH + 1) / 2, T>(S, g, hx, ht, t, sc, sd, with_dot);
+      case 4: return stage_coeffs_q<4, (CH + 1) / 2, T>(S, g, hx, ht, t, sc, sd, with_dot);
+      default: break;
+    }
+  }
   if (M == 1) {
     // c(t) = c0 + t c1, dc/dt = Ht[j][0]
 #pragma unroll kTermUnroll<T>
@@ -1014,8 +1076,9 @@ enum : int {
 // of the groups of a wave changes.  c(t) is staged in LDS only when the
 // group's evaluation point moves (RK4 stages 0, 1 and 3; the corrector
 // reuses stage 3's c(t + dt)).
-// REP: the segment runs R > 1 replicas per path (Seg::R; launches of few
-// paths), compiled apart so that the loop of R = 1 carries none of it.
+// REP: the latency instance (launches of few paths, one wave per SIMD at
+// most): R >= 1 replicas per path (Seg::R) and chunked coefficient staging,
+// compiled apart so that the throughput loop carries none of it.
 // HOIST: stage_coeffs requests all terms of a coefficient at once (orders
 // M <= 4; up to n = 16, outside the multi-size kernel, whose registers
 // would otherwise cost it occupancy).
@@ -1167,7 +1230,7 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
       const double te = stg ? ts + a : (pe == kPhNewton ? t : 1.0);
       const cx<double> xe = stg ? cmk<double>(xl.re + a * kp.re, xl.im + a * kp.im) : x;
       const bool restage = act && te != tst;
-      if (__any(restage)) stage_coeffs<double, HOIST>(Sk, g, restage, inst, te, L.coef(), !gdot);
+      if (__any(restage)) stage_coeffs<double, HOIST, REP ? 4 : 0>(Sk, g, restage, inst, te, L.coef(), !gdot);
       if (restage) tst = te;
       publish_x<double>(sx, xe, g, n, act, row);
       HCX_WAVE_SYNC();
@@ -1341,9 +1404,9 @@ void track_kernel(const Launch LN) {
   }
 }
 
-// FP64 launches of one segment with R > 1 replicas per path (launches of
-// few paths, latency bound: one wave per SIMD at most, so the register
-// budget is a full SIMD's).
+// Latency instance: FP64 launches of one segment and few paths (one wave
+// per SIMD at most, so the register budget is a full SIMD's), with R >= 1
+// replicas per path.
 template <int NMAX, int DEG, bool EXACT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
 void track_kernel_rep(const Launch LN) {

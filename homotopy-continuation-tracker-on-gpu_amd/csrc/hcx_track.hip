@@ -1253,14 +1253,16 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
       // split saves
       while (2 * R * S.G <= 64 && 8 * R <= p->htT && waves(2 * R) <= 4LL * g_cu) R *= 2;
     }
-    const KernelFn rf = R > 1 ? pick_rep(pl.nmax, pl.deg, pl.exact) : nullptr;
+    // the latency instance (its register budget is a whole SIMD's), also
+    // for R = 1
+    const KernelFn rf = pick_rep(pl.nmax, pl.deg, pl.exact);
     if (!rf) R = 1;
     const size_t regions = (size_t)(64 / (S.G * R)) * (size_t)lg;
     if (regions + (size_t)pl.tab_lds + terms <= 65536) {
       LN.dot_lds = 1;
       S.lds_group = lg;
       S.R = R;
-      if (R > 1) pl.fn = rf;
+      if (rf) pl.fn = rf;
       if (!pl.exact) pl.tab_lds = (int)terms;
       seg_groups[0] = waves(R);
       total_groups_needed = seg_groups[0];
