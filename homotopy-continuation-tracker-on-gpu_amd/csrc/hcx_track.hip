@@ -317,10 +317,10 @@ static KernelFn pick_kernel(int nmax, int deg, int mode) {
 // The replica instance of a launch planned on `pl` (nullptr: none).
 static KernelFn pick_rep(int nmax, int deg, bool exact) {
   if (exact) {
-    if (deg > 3) return nullptr;
+    if (deg > 4) return nullptr;
     deg = deg < 2 ? 2 : deg;
-    if (nmax == 3 || nmax == 5 || nmax == 6) return kernel_rep_xs(nmax, deg);
-    if (nmax == 9 || nmax == 12 || nmax == 16) return kernel_rep_xl(nmax, deg);
+    if (nmax >= 3 && nmax <= 6) return kernel_rep_xs(nmax, deg);
+    if (nmax == 8 || nmax == 9 || nmax == 12 || nmax == 16) return kernel_rep_xl(nmax, deg);
     return nullptr;
   }
   switch (nmax) {
@@ -332,12 +332,15 @@ static KernelFn pick_rep(int nmax, int deg, bool exact) {
 }
 
 static KernelFn pick_exact(int n, int deg, int mode) {
-  if (deg > 3) return nullptr;
+  // degree 4 (8-byte compact terms of four variables): n = 4, 5 in FP64 mode
+  if (deg > 4 || (deg == 4 && mode != 0)) return nullptr;
   deg = deg < 2 ? 2 : deg;
   switch (n) {
     case 3: return mode == 0 ? kernel_x3_m0(deg) : kernel_x3_m1(deg);
+    case 4: return mode == 0 ? kernel_x4_m0(deg) : kernel_x4_m1(deg);
     case 5: return mode == 0 ? kernel_x5_m0(deg) : kernel_x5_m1(deg);
     case 6: return mode == 0 ? kernel_x6_m0(deg) : kernel_x6_m1(deg);
+    case 8: return mode == 0 ? kernel_x8_m0(deg) : kernel_x8_m1(deg);
     case 9: return mode == 0 ? kernel_x9_m0(deg) : kernel_x9_m1(deg);
     case 12: return mode == 0 ? kernel_x12_m0(deg) : kernel_x12_m1(deg);
     case 16: return mode == 0 ? kernel_x16_m0(deg) : kernel_x16_m1(deg);

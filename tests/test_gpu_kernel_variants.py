@@ -120,3 +120,32 @@ def test_replica_groups_match_whole_sums(hcx_mod, gpu, oracle_mod, replicas):
     assert e2.max() <= 1e-10
     torch.cuda.synchronize()
     prob.close()
+
+
+@pytest.mark.parametrize("name", ["distorted_2view_triangulation", "optimal_PnP_quat"])
+def test_degree4_instance_equals_generic(hcx_mod, gpu, name):
+    """Terms of degree 4 at n = 5 and n = 4 (two of the reference's problems):
+    the size-specialised degree-4 instance (8-byte compact terms in LDS) and
+    the generic instance (degree-6 records whose missing factors read the
+    constant 1) give the same flags and step counts and the same roots to
+    1e-12 relative, in the throughput and the latency instances."""
+    from test_reference_problems import load
+    hcx = hcx_mod
+    d = load(name)
+    prob = hcx.Problem(d["n"], d["C"], d["hx4"], d["ht3"], d["start_sols"], max_order_t=d["M"], cmap=d["cmap"],
+                       p0=d["start_params"])
+    tp = np.vstack([d["target_params"].reshape(1, -1), hcx.random_cn(5, 2 * d["k"]).reshape(2, -1)])
+    b = hcx.Batch(prob, 3, target_params=tp)
+    kw = dict(precision=0, max_steps=d["max_steps"], max_corr=3, incr_steps=d["incr_steps"])
+    for reps in (1, 0):
+        a = _run(hcx, prob, b, specialize=1, replicas=reps, **kw)
+        g = _run(hcx, prob, b, specialize=0, lanes_per_path=d["n"], replicas=reps, **kw)
+        assert (a["flags"] == g["flags"]).all(), (name, reps)
+        assert (a["steps"] == g["steps"]).all(), (name, reps)
+        ok = (a["flags"] & hcx.FLAG_REFINED) != 0
+        assert ok.mean() > 0.5
+        err = np.abs(a["roots"][ok] - g["roots"][ok]).max(axis=1) / np.maximum(
+            1.0, np.abs(g["roots"][ok]).max(axis=1))
+        print(f"[{name} replicas={reps}] refined {int(ok.sum())}, max rel diff {err.max():.2e}", flush=True)
+        assert err.max() <= 1e-12
+    prob.close()
