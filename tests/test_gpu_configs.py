@@ -217,11 +217,12 @@ def test_six_served_types_vs_oracle(hcx_mod, oracle_mod, six_types):
     srv.close()
 
 
-def test_large_types_alone_equal_in_mix(hcx_mod, six_types):
+@pytest.mark.parametrize("prec", [1, 0])
+def test_large_types_alone_equal_in_mix(hcx_mod, six_types, prec):
     from hcx import workloads
     hcx = hcx_mod
     types, probs = six_types
-    srv = hcx.Server(hcx.default_opts(precision=hcx.PREC_MIXED), hbm_budget_bytes=512 << 20, max_batch_requests=16)
+    srv = hcx.Server(hcx.default_opts(precision=prec), hbm_budget_bytes=512 << 20, max_batch_requests=16)
     tids = [srv.add_type(p) for p in probs]
     probe = {ti: workloads.targets(types[ti]["k"], 1, 700 + ti)[0] for ti in (3, 4, 5)}  # n = 12, 16, 30
     alone = {ti: wait(srv, srv.submit(tids[ti], x), tids[ti]) for ti, x in probe.items()}

@@ -27,11 +27,15 @@ def wait(server, rid, tid, timeout=60.0):
         time.sleep(0.001)
 
 
-def test_alone_equals_in_mix(hcx_mod, served):
+@pytest.mark.parametrize("prec", [1, 0])
+def test_alone_equals_in_mix(hcx_mod, served, prec):
+    """A request's result is the same bit for bit alone and in a mix, in
+    mixed precision and in FP64 (where a lone request's launch holds few
+    paths and runs the latency instance, a mixed batch the throughput one)."""
     from hcx import workloads
     hcx = hcx_mod
     types, probs = served
-    srv = hcx.Server(hcx.default_opts(precision=hcx.PREC_MIXED), hbm_budget_bytes=256 << 20)
+    srv = hcx.Server(hcx.default_opts(precision=prec), hbm_budget_bytes=256 << 20)
     tids = [srv.add_type(p) for p in probs]
     x = workloads.targets(types[2]["k"], 1, 123)[0]
     alone = wait(srv, srv.submit(tids[2], x), tids[2])
