@@ -182,6 +182,8 @@ def main():
                          "per launch, beside the reference's published GPU times)")
     ap.add_argument("--problems", default="",
                     help="config 7: comma-separated problem names (default: all with a published time)")
+    ap.add_argument("--replicas", type=int, default=0,
+                    help="config 7: replica lane groups per path (0 = the library's choice, 1 = off)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -858,7 +860,7 @@ def bench_reference_problems(args):
         for label, static, prec in (("fp64_per_problem", True, hcx.PREC_FP64),
                                     ("fp64_persistent", False, hcx.PREC_FP64)):
             o = hcx.default_opts(precision=prec, max_steps=int(d["max_steps"]), max_corr=int(d["max_corr"]),
-                                 incr_steps=int(d["incr_steps"]))
+                                 incr_steps=int(d["incr_steps"]), replicas=args.replicas)
             r = hcx.track(prob, b, o, static=static, stream=stream)
             torch.cuda.synchronize()
             ts = []

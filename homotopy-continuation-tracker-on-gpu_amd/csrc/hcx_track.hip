@@ -1229,8 +1229,8 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
   // no other waves hide the latency of its evaluations.  dc/dt then goes to
   // LDS next to c(t), the generic instances read their Term records from an
   // LDS copy, and each path gets R replicas of its lane group (the most, a
-  // power of two with R * G <= 64 and at least 4 dH/dt terms per replica,
-  // that keep the launch within one wave per SIMD; or o->replicas), which
+  // power of two up to 8 with R * G <= 64, that keep the launch within one
+  // wave per SIMD, or half of one for light rows; or o->replicas), which
   // split the evaluation's term sums, when the group regions and the records
   // fit in 64 KiB.
   if (o->precision == HCX_PREC_FP64 && nseg == 1 && (total_groups_needed <= 4LL * g_cu || o->replicas > 1)) {
@@ -1251,10 +1251,16 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
         return HCX_ERR_INVALID_ARG;
       }
     } else {
-      // every replica keeps at least 4 terms of the row's dH/dt sum (the
-      // longest single sum): below that the butterfly costs more than the
-      // split saves
-      while (2 * R * S.G <= 64 && 8 * R <= p->htT && waves(2 * R) <= 4LL * g_cu) R *= 2;
+      // at most 8 replicas; beyond half a wave per SIMD only for rows whose
+      // evaluation outweighs the butterfly (complex products per row >= 8
+      // per exchanged sum: measured on the reference's problems, where
+      // 583 paths of 46 quadratic terms per row ran slower at R = 4 than at
+      // R = 2 and 668 paths of 84 cubic terms faster)
+      const int dg = std::max(p->deg, 1);
+      const double row_products = (double)p->n * p->hxT * (dg - 1) + (double)p->htT * dg;
+      const bool heavy = row_products >= 8.0 * (p->n + 1);
+      while (R < 8 && 2 * R * S.G <= 64 && waves(2 * R) <= 4LL * g_cu && (heavy || waves(2 * R) <= 2LL * g_cu))
+        R *= 2;
     }
     // the latency instance (its register budget is a whole SIMD's), also
     // for R = 1
