@@ -1469,7 +1469,7 @@ HCX_GENERIC_UNITS(32, 0)
 HCX_GENERIC_UNITS(32, 1)
 #undef HCX_GENERIC_UNITS
 // Size-specialised instances (n = G = N) for the problem sizes of the
-// benchmark workloads and of the reference's small problems (n = 4, 8),
+// benchmark workloads and of the reference's small problems (n = 4, 8, 10),
 // DEG 2 and 3; DEG 4 for n = 4, 5 in FP64 mode.
 // Multi-size instances (n in {3, 5, 6, 9, 12}, DEG 2) for uneven batches.
 KernelFn kernel_set_m0();
@@ -1486,6 +1486,8 @@ KernelFn kernel_x8_m0(int deg);
 KernelFn kernel_x8_m1(int deg);
 KernelFn kernel_x9_m0(int deg);
 KernelFn kernel_x9_m1(int deg);
+KernelFn kernel_x10_m0(int deg);
+KernelFn kernel_x10_m1(int deg);
 KernelFn kernel_x12_m0(int deg);
 KernelFn kernel_x12_m1(int deg);
 KernelFn kernel_x16_m0(int deg);
@@ -1493,8 +1495,8 @@ KernelFn kernel_x16_m1(int deg);
 KernelFn kernel_x30_m0(int deg);
 KernelFn kernel_x30_m1(int deg);
 // Latency instances (FP64, one segment, few paths): size-specialised for
-// n in {3, 4, 5, 6} and {8, 9, 12, 16} (DEG 2 and 3), generic for NMAX 8, 16
-// and 32 (DEG 2, 3 and 6).
+// n in {3, 4, 5, 6} and {8, 9, 10, 12, 16} (DEG 2 and 3), generic for NMAX 8,
+// 16 and 32 (DEG 2, 3 and 6).
 KernelFn kernel_rep_xs(int n, int deg);
 KernelFn kernel_rep_xl(int n, int deg);
 KernelFn kernel_rep_g8(int deg);

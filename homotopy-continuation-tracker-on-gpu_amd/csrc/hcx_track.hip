@@ -320,7 +320,7 @@ static KernelFn pick_rep(int nmax, int deg, bool exact) {
     if (deg > 4) return nullptr;
     deg = deg < 2 ? 2 : deg;
     if (nmax >= 3 && nmax <= 6) return kernel_rep_xs(nmax, deg);
-    if (nmax == 8 || nmax == 9 || nmax == 12 || nmax == 16) return kernel_rep_xl(nmax, deg);
+    if (nmax == 8 || nmax == 9 || nmax == 10 || nmax == 12 || nmax == 16) return kernel_rep_xl(nmax, deg);
     return nullptr;
   }
   switch (nmax) {
@@ -342,6 +342,7 @@ static KernelFn pick_exact(int n, int deg, int mode) {
     case 6: return mode == 0 ? kernel_x6_m0(deg) : kernel_x6_m1(deg);
     case 8: return mode == 0 ? kernel_x8_m0(deg) : kernel_x8_m1(deg);
     case 9: return mode == 0 ? kernel_x9_m0(deg) : kernel_x9_m1(deg);
+    case 10: return mode == 0 ? kernel_x10_m0(deg) : kernel_x10_m1(deg);
     case 12: return mode == 0 ? kernel_x12_m0(deg) : kernel_x12_m1(deg);
     case 16: return mode == 0 ? kernel_x16_m0(deg) : kernel_x16_m1(deg);
     case 30: return mode == 0 ? kernel_x30_m0(deg) : kernel_x30_m1(deg);
