@@ -1227,14 +1227,14 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
     total_groups_needed += seg_groups[s];
   }
   // A one-segment FP64 launch of at most one wave per SIMD is latency bound:
-  // no other waves hide the latency of its evaluations.  dc/dt then goes to
-  // LDS next to c(t), the generic instances read their Term records from an
-  // LDS copy, and each path gets R replicas of its lane group (the most, a
-  // power of two up to 8 with R * G <= 64, that keep the launch within one
-  // wave per SIMD, or half of one above R = 2 for light rows; or
-  // o->replicas), which
-  // split the evaluation's term sums, when the group regions and the records
-  // fit in 64 KiB.
+  // no other waves hide the latency of its evaluations.  It runs the latency
+  // instance of its size when one exists; dc/dt goes to LDS next to c(t),
+  // the generic instances read their Term records from an LDS copy, and
+  // each path gets R replicas of its lane group, which split the
+  // evaluation's term sums (the most, a power of two up to 8 with R * G <=
+  // 64, that keep the launch within one wave per SIMD, or half of one above
+  // R = 2 for light rows; or o->replicas).  All of it only when the group
+  // regions and the records fit in 64 KiB.
   if (o->precision == HCX_PREC_FP64 && nseg == 1 && (total_groups_needed <= 4LL * g_cu || o->replicas > 1)) {
     Seg& S = LN.seg[0];
     const hcx_problem* p = probs[0];

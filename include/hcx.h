@@ -85,9 +85,10 @@ typedef struct {
                              0 (default) = FP64 corrector at every t, so step acceptance is decided in FP64 */
   double real_ref_tol;    /* HCX_FLAG_REAL_REF: the reference's real test on the tracked root,
                              ZERO_IMAG_PART_TOL_FOR_SP = 1e-4 (definitions.hpp:25, Evaluations.cpp:96-104) */
-  int replicas;           /* FP64, one-segment launches: R copies of each path's lane group, each summing
-                             every R-th term of the evaluation (a power of two, R * lanes <= 64); 0 = automatic
-                             (the most that keep a launch of few paths within one wave per SIMD), 1 = off */
+  int replicas;           /* FP64, one-segment launches on sizes with a latency instance: R copies of each
+                             path's lane group, each summing every R-th term of the evaluation (a power of
+                             two, R * lanes <= 64); 0 = automatic (launches of at most one wave per SIMD),
+                             1 = off */
 } hcx_track_opts;
 
 void hcx_track_opts_default(hcx_track_opts* o);
