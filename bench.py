@@ -173,6 +173,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=800.0,
                     help="default run: the fixed Poisson request rate the latency is stated at")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-refprob", action="store_true",
+                    help="skip the reference's problems one instance per launch in the headline run")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="end with one 256 MiB device copy, the reference dispatch for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE scale factors")
@@ -362,6 +364,19 @@ def main():
     if not args.no_serve:
         serving = serving_leg(args)
 
+    refprob = None
+    if not args.no_refprob:
+        log("bench: the reference's problems, one instance per launch")
+        rows = reference_problem_rows(args, 5)
+        refprob = {"faster_than_published": sum(r["fp64_per_problem"]["ms_p50"] < r["reference_gpu_hc_ms"]
+                                                for r in rows),
+                   "problems": len(rows),
+                   "ms": {r["problem"]: [round(r["fp64_per_problem"]["ms_p50"], 3), r["reference_gpu_hc_ms"]]
+                          for r in rows},
+                   "note": "[hcx FP64 ms p50, reference GPU-HC ms]: one target instance per launch (P2C + "
+                           "tracker, the reference's settings) against the reference's published times "
+                           "(README.md:45-63, complex FP32, GPU not stated); bench.py --config 7 for details"}
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -397,6 +412,7 @@ def main():
         "parity_batched_vs_fp64_per_problem": parity,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "reference_problems_one_instance": refprob,
     }
     print(json.dumps(out))
     if args.pmc_calibrate:
@@ -831,12 +847,19 @@ def bench_reference_problems(args):
     time, synchronous), FP64, timed with HIP events around P2C + tracker;
     beside each, the reference's published GPU-HC time for the same problem
     and path count (complex FP32, GPU not stated)."""
+    rows = reference_problem_rows(args, max(5, args.steps))
+    print(json.dumps({"metric": METRIC, "config": {"workload": "the reference's problems, one instance per "
+                                                               "launch, reference settings, FP64"},
+                      "problems": rows,
+                      "note": "reference times: README.md:45-63, complex FP32 on an unstated GPU"}))
+
+
+def reference_problem_rows(args, reps):
     import numpy as np
     import torch
     import hcx
     fix = os.path.join(ROOT, "tests", "golden", "reference")
     stream = torch.cuda.current_stream()
-    reps = max(5, args.steps)
     rows = []
     only = set(filter(None, args.problems.split(",")))
     for name, (title, paths, ref_ms) in REFERENCE_README_MS.items():
@@ -884,10 +907,7 @@ def bench_reference_problems(args):
             f"(persistent {out['fp64_persistent']['ms_p50']:.3f} ms) vs reference {ref_ms} ms")
         del b
         prob.close()
-    print(json.dumps({"metric": METRIC, "config": {"workload": "the reference's problems, one instance per "
-                                                               "launch, reference settings, FP64"},
-                      "problems": rows,
-                      "note": "reference times: README.md:45-63, complex FP32 on an unstated GPU"}))
+    return rows
 
 
 if __name__ == "__main__":
