@@ -43,6 +43,7 @@ def test_default_options_are_reference_constants(hcx_mod):
     assert o.unique_tol == 1e-4  # definitions.hpp:24
     assert o.real_tol == 1e-8  # definitions.hpp:26 (DP)
     assert o.real_ref_tol == 1e-4  # definitions.hpp:25 (SP), the reference's own real test
+    assert o.replicas == 0  # replica lane groups chosen by the library
 
 
 def test_no_device_fails_loudly_without_gpu(hcx_mod):

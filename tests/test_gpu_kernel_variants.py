@@ -149,3 +149,16 @@ def test_degree4_instance_equals_generic(hcx_mod, gpu, name):
         print(f"[{name} replicas={reps}] refined {int(ok.sum())}, max rel diff {err.max():.2e}", flush=True)
         assert err.max() <= 1e-12
     prob.close()
+
+
+def test_replicas_option_checked(hcx_mod, gpu):
+    """replicas must be a power of two whose groups fit a wave."""
+    from hcx import workloads
+    hcx = hcx_mod
+    ty, prob = _type(hcx, 6, 64)
+    b = hcx.Batch(prob, 1, target_params=workloads.targets(ty["k"], 1, seed=5))
+    with pytest.raises(hcx.HcxError, match="invalid tracker options"):
+        hcx.track(prob, b, hcx.default_opts(replicas=3))
+    with pytest.raises(hcx.HcxError, match="exceed a wave"):
+        hcx.track(prob, b, hcx.default_opts(replicas=16))
+    prob.close()
