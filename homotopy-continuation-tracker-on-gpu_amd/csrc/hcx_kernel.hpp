@@ -1080,9 +1080,9 @@ enum : int {
 // most): R >= 1 replicas per path (Seg::R) and chunked coefficient staging,
 // compiled apart so that the throughput loop carries none of it.
 // HOIST: stage_coeffs requests all terms of a coefficient at once (orders
-// M <= 4; up to n = 16, outside the multi-size kernel, whose registers
-// would otherwise cost it occupancy).
-template <int NMAX, int DEG, bool EXACT, bool REP, bool HOIST = (NMAX <= 16)>
+// M <= 4; size-specialised instances up to n = 16 outside the multi-size
+// kernel: the others would lose occupancy to the registers).
+template <int NMAX, int DEG, bool EXACT, bool REP, bool HOIST = (EXACT && NMAX <= 16)>
 __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsigned char* smem, int lane) {
   Seg S = LN.seg[si];
   constexpr int GE = exact_lanes(NMAX);
@@ -1101,12 +1101,12 @@ __device__ __forceinline__ void run_segment_fp64(const Launch& LN, int si, unsig
   const int gle = row ? g.gl : 0;
   constexpr bool LT = EXACT && (NMAX <= 16 || DEG <= 2);
   const void* lt = nullptr;
-  // tl: the generic instance's full Term records are in LDS (launches of
-  // few paths; tab_lds then holds their bytes)
+  // tl: the generic latency instance's full Term records are in LDS
+  // (tab_lds then holds their bytes)
   bool tl = false;
   if constexpr (LT) {
     lt = stage_tables<DEG>(S, n, smem);
-  } else if (!EXACT) {
+  } else if constexpr (!EXACT && REP) {
     if (LN.tab_lds > 0) {
       lt = stage_terms(S, n, smem);
       tl = true;

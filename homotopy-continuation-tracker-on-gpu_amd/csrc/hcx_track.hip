@@ -1276,7 +1276,8 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
       S.lds_group = lg;
       S.R = R;
       if (rf) pl.fn = rf;
-      if (!pl.exact) pl.tab_lds = (int)terms;
+      // the generic latency instances read their term records from LDS
+      if (!pl.exact && rf) pl.tab_lds = (int)terms;
       seg_groups[0] = waves(R);
       total_groups_needed = seg_groups[0];
     }
