@@ -82,29 +82,34 @@ def test_n30_dpp_groups(hcx_mod, gpu, prec):
     prob.close()
 
 
+@pytest.mark.parametrize("generic", [False, True])
 @pytest.mark.parametrize("replicas", [2, 4, 8])
-def test_replica_groups_match_whole_sums(hcx_mod, gpu, oracle_mod, replicas):
+def test_replica_groups_match_whole_sums(hcx_mod, gpu, oracle_mod, replicas, generic):
     """FP64 launches of few paths split every term sum of the evaluation over
     R replica groups of each path (each replica sums every R-th term, a
     butterfly adds the partial sums).  Only the summation order changes: the
     flags and step counts of every path equal those of the launch with whole
     sums per lane (replicas = 1), and the refined roots agree to 1e-10
-    relative, as they do with the oracle's."""
+    relative, as they do with the oracle's.  Size-specialised instance (6
+    lanes per replica) and generic instance (8 lanes per replica, two of
+    them without a row)."""
     import torch
     from hcx import workloads
     hcx, oracle = hcx_mod, oracle_mod
     ty = workloads.quadratic_types([(6, 64)], seed=9)[0]
     prob = hcx.Problem.from_system(ty["system"], ty["start"], p0=ty["p0"])
     b = hcx.Batch(prob, 2, target_params=workloads.targets(ty["k"], 2, seed=78))
-    one = _run(hcx, prob, b, precision=0, replicas=1)
-    rep = _run(hcx, prob, b, precision=0, replicas=replicas)
+    shape = dict(specialize=0, lanes_per_path=8) if generic else {}
+    one = _run(hcx, prob, b, precision=0, replicas=1, **shape)
+    rep = _run(hcx, prob, b, precision=0, replicas=replicas, **shape)
     assert (one["flags"] == rep["flags"]).all()
     assert (one["steps"] == rep["steps"]).all()
     ok = (one["flags"] & hcx.FLAG_REFINED) != 0
     assert ok.mean() > 0.9
     err = np.abs(one["roots"][ok] - rep["roots"][ok]).max(axis=1) / np.maximum(
         1.0, np.abs(one["roots"][ok]).max(axis=1))
-    print(f"[replicas={replicas}] refined {int(ok.sum())}, max rel diff {err.max():.2e}", flush=True)
+    print(f"[replicas={replicas} generic={generic}] refined {int(ok.sum())}, max rel diff {err.max():.2e}",
+          flush=True)
     assert err.max() <= 1e-10
     # and against the oracle on the same paths
     sysf = ty["system"]
