@@ -1232,7 +1232,7 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
   // the generic instances read their Term records from an LDS copy, and
   // each path gets R replicas of its lane group, which split the
   // evaluation's term sums (the most, a power of two up to 8 with R * G <=
-  // 64, that keep the launch within one wave per SIMD, or half of one above
+  // 64, that keep the launch within two waves per SIMD, or half of one above
   // R = 2 for light rows; or o->replicas).  All of it only when the group
   // regions and the records fit in 64 KiB.
   if (o->precision == HCX_PREC_FP64 && nseg == 1 && (total_groups_needed <= 4LL * g_cu || o->replicas > 1)) {
@@ -1257,12 +1257,13 @@ hcx_status track_impl(int nseg, hcx_problem* const* probs, const hcx_batch_desc*
       // whose evaluation outweighs the butterfly (complex products per row
       // >= 8 per exchanged sum).  Measured: 583 paths of 46 quadratic terms
       // per row ran slower at R = 4 than at R = 2, 668 paths of 84 cubic
-      // terms faster, and 4,096 paths of a light n = 6 type faster at R = 2
-      // (820 waves) than at R = 1
+      // terms faster, 4,096 paths of a light n = 6 type faster at R = 2 (820
+      // waves) than at R = 1, and 2,560 paths at n = 16 faster at R = 2
+      // (1,280 waves) than at R = 1
       const int dg = std::max(p->deg, 1);
       const double row_products = (double)p->n * p->hxT * (dg - 1) + (double)p->htT * dg;
       const bool heavy = row_products >= 8.0 * (p->n + 1);
-      while (R < 8 && 2 * R * S.G <= 64 && waves(2 * R) <= 4LL * g_cu &&
+      while (R < 8 && 2 * R * S.G <= 64 && waves(2 * R) <= 8LL * g_cu &&
              (R == 1 || heavy || waves(2 * R) <= 2LL * g_cu))
         R *= 2;
     }
